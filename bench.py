@@ -1,0 +1,190 @@
+"""Benchmark of the CGFDM time step on one MI355X.
+
+Workload (N=1): CGFDM curvilinear grid 512 x 512 x 256 at h = 100 m, eight seeded
+Gaussian hills (0.5-2 km tall, 3-15 km wide), four-layer medium with a 5%
+Gaussian random perturbation (1 km correlation), 20-layer CFS-PML on five faces,
+traction-image free surface, point moment source, 400 surface stations and the
+PGV map, FP32 wavefield. One "step" is one full time step: source injection,
+four RK4 stages (derivatives, free surface, PML, update) and the observers.
+Inputs are resident in HBM before the timed region starts.
+
+Prints one JSON line (see the keys below). Per-kernel times come from HIP events
+recorded on the solver's stream inside the timed region.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cgfdm3d-eqr-sc_amd"))
+
+METRIC = "Gcell-updates/s & %HBM peak, FP32 fwd and fwd+adjoint vs plain, at stated L2 err"
+HBM_PEAK_GBS = 8000.0
+W_BYTES = 36          # 9 fields x 4 B
+C_BYTES = {0: 48, 1: 24, 2: 24}  # 12 coefficient planes per cell (fp32 / fp16 / bf16)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stage_bytes(coef_storage):
+    """Algorithmic HBM bytes per cell per RK stage of the fused interior kernel:
+    stage 0 and 3 read W and write two arrays (3w + c), stages 1 and 2 read W, the
+    RK base and the accumulator and write two arrays (5w + c). Mean over stages."""
+    c = C_BYTES[coef_storage]
+    return (2 * (3 * W_BYTES + c) + 2 * (5 * W_BYTES + c)) / 4.0
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch of the interior kernel from the committed rocprof
+    FETCH_SIZE / WRITE_SIZE summary, if present."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return d.get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(model, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    log(f"cpu baseline: oracle FP32 on {threads} threads, {steps} steps of the same grid")
+    o = oracle.Oracle(model, prec=4, threads=threads)
+    o.step(1)  # first touch
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    del o
+    return model.cells * steps / dt / 1e9, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--coef", type=int, default=0, help="0 fp32, 1 fp16, 2 bf16 coefficient storage")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: the single-rank domain holds one GPU's full share; "
+              "multi-GPU sharding is a design note, not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from cgfdm_amd import model as M
+    from cgfdm_amd import solver as S
+
+    t0 = time.perf_counter()
+    m = M.config2(nt=args.warmup + args.steps + 1, nx=args.nx, ny=args.ny, nz=args.nz)
+    log(f"model {m.nx}x{m.ny}x{m.nz} built in {time.perf_counter() - t0:.1f}s, dt={m.dt:.4e}")
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    g = S.Solver(m, precision=4, coef_storage=args.coef)
+    g.set_stream(stream)
+    log(f"solver ready: {g.device_bytes() / 2**30:.1f} GiB on device")
+    g.step(args.warmup)
+    g.synchronize()
+
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    g.timing(True)
+    ev0.record(stream)
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    ev1.record(stream)
+    g.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tm = g.timing(False)
+    ev_ms = ev0.elapsed_time(ev1)
+    log(f"timed {args.steps} steps: wall {wall:.3f}s, events {ev_ms / 1e3:.3f}s; kernels {tm}")
+
+    ms_step = wall * 1e3 / args.steps
+    cells = m.cells
+    gcells = cells * args.steps / wall / 1e9
+
+    # dominant kernel: fused interior stage
+    launch_ms = tm["interior_ms"] / max(tm["interior_n"], 1)
+    cells_launch = m.nx * m.ny * (m.nz - 3)
+    alg_bytes = stage_bytes(args.coef) * cells_launch
+    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+    step_bytes = 4 * stage_bytes(args.coef) * cells
+    hbm_step = step_bytes / (ms_step * 1e-3) / 1e9
+    traffic = traffic_from_profiles("traffic_interior_config2.json")
+
+    # accuracy leg: FP32 vs FP64 GPU on the 128 x 128 x 64 half-space (config 1), stations
+    l2 = None
+    if not args.no_accuracy:
+        m1 = M.config1(nt=400)
+        a = S.Solver(m1, precision=4, coef_storage=args.coef)
+        a.step(400)
+        b = S.Solver(m1, precision=8)
+        b.step(400)
+        sa, sb = a.stations(), b.stations()
+        l2 = float(np.linalg.norm(sa[:, :, :3] - sb[:, :, :3]) / np.linalg.norm(sb[:, :, :3]))
+        a.close()
+        b.close()
+        log(f"accuracy: config1 400 steps FP32 vs FP64 station velocity rel L2 = {l2:.3e}")
+
+    g.close()
+    torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        v, secs = cpu_baseline(m, args.cpu_steps, threads)
+        cpu = {"value": v, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"CPU oracle (C++ restatement, FP32, OpenMP) on the same {m.nx}x{m.ny}x{m.nz} grid, "
+                         f"{args.cpu_steps} full steps ({secs:.1f}s)"}
+        log(f"cpu baseline {v:.4f} Gcell/s")
+
+    dtype = {0: "f32", 1: "f32 (fp16 coefficients)", 2: "f32 (bf16 coefficients)"}[args.coef]
+    out = {
+        "metric": METRIC,
+        "value": gcells,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {"workload": f"CGFDM curvilinear {m.nx}x{m.ny}x{m.nz} h=100m, 8 Gaussian hills, "
+                               f"4 layers + 5% random, PML 20, free surface, point source, "
+                               f"{len(m.stations)} stations + PGV, RK4",
+                   "cells": cells, "dt": m.dt},
+        "hbm_frac_step": hbm_step / HBM_PEAK_GBS,
+        "l2_err_fp32_vs_fp64": l2,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "cg_interior", "launch_ms": launch_ms,
+                     "alg_bytes_per_cell": stage_bytes(args.coef), "cells_per_launch": cells_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,304 @@
+"""Model description and synthetic inputs for the CGFDM / ME-AWENO solvers.
+
+A `Model` carries exactly what the reference's setup stage hands to its time
+loop (reference run.cpp:28-138): curvilinear grid coordinates on the padded
+grid, the medium (Vs, Vp, rho) on the padded grid, the finite-fault moment-rate
+table and the station list.  The builders below follow the reference's own
+constructions:
+
+* flat grid            coord.cpp:100-135   (construct_flat_coord)
+* terrain-following z  coord.cpp:205-257   (construct_terrain_coord)
+* layered medium       medium.cpp:89-213   (generate_1D_medium)
+* moment tensor        multiSource.cpp:450-535 (solveMomentRate)
+* Ricker-type rate     singleSource.cpp:42-68  (sourceFunction / 1e9)
+* CFL time step        calc_CFL.cpp:255-349 (Hmin by point-to-plane distance)
+
+Arrays are float32, SoA, x fastest: coord[3, NZ, NY, NX] with NX = nx + 6
+(HALO = 3 on every side, reference macro.h:39).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Optional
+
+import numpy as np
+
+HALO = 3
+
+
+@dataclasses.dataclass
+class Model:
+    nx: int
+    ny: int
+    nz: int
+    dh: float
+    dt: float
+    nt: int                      # time steps the station buffers hold
+    npml: int
+    coord: np.ndarray            # float32 [3, NZ, NY, NX]
+    medium: np.ndarray           # float32 [3, NZ, NY, NX]  (Vs, Vp, rho)
+    free_surface: bool = True
+    src_cells: Optional[np.ndarray] = None   # int64 [np] padded linear cell index
+    src_rate: Optional[np.ndarray] = None    # float32 [np, src_nt, 6] = M_ij * area * slip-rate
+    src_dt: float = 0.0
+    stations: Optional[np.ndarray] = None    # int32 [ns, 3] padded (i, j, k)
+    name: str = ""
+    terrain: Optional[np.ndarray] = None     # float32 [NY, NX] surface elevation
+
+    @property
+    def shape_padded(self):
+        return (self.nz + 2 * HALO, self.ny + 2 * HALO, self.nx + 2 * HALO)
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def cell_index(self, i, j, k):
+        NZ, NY, NX = self.shape_padded
+        return np.int64(i) + np.int64(NX) * (np.int64(j) + np.int64(NY) * np.int64(k))
+
+
+# ---------------------------------------------------------------------------
+# grids
+# ---------------------------------------------------------------------------
+def flat_coord(nx, ny, nz, dh, center_x=0, center_y=0):
+    """coord.cpp:100-135: x = (I-H)*DH - cX*DH, z = (K-H+1)*DH - NZ*DH (single rank)."""
+    NX, NY, NZ = nx + 2 * HALO, ny + 2 * HALO, nz + 2 * HALO
+    I = np.arange(NX, dtype=np.float64)
+    J = np.arange(NY, dtype=np.float64)
+    K = np.arange(NZ, dtype=np.float64)
+    c = np.empty((3, NZ, NY, NX), np.float32)
+    c[0] = ((I - HALO) * dh - center_x * dh)[None, None, :]
+    c[1] = ((J - HALO) * dh - center_y * dh)[None, :, None]
+    c[2] = ((K - HALO + 1) * dh - nz * dh)[:, None, None]
+    terrain = np.zeros((NY, NX), np.float32)
+    return c, terrain
+
+
+def terrain_coord(nx, ny, nz, dh, terrain, depth, center_x=0, center_y=0):
+    """coord.cpp:205-257: z = -|D| + (h + |D|)/(NZ-1) * (k - H), x/y as the flat grid."""
+    c, _ = flat_coord(nx, ny, nz, dh, center_x, center_y)
+    NZ = nz + 2 * HALO
+    K = np.arange(NZ, dtype=np.float64) - HALO
+    dz = (terrain.astype(np.float64) + abs(depth)) / (nz - 1)
+    c[2] = (-abs(depth) + dz[None, :, :] * K[:, None, None]).astype(np.float32)
+    return c
+
+
+def gaussian_hills(nx, ny, dh, npml, nhills, seed, hmin=500.0, hmax=2000.0,
+                   smin=3000.0, smax=15000.0, center_x=0, center_y=0):
+    """Sum of seeded Gaussian hills h*exp(-r^2/(2 s^2)), centres inside the PML frame."""
+    rng = np.random.default_rng(seed)
+    NX, NY = nx + 2 * HALO, ny + 2 * HALO
+    x = (np.arange(NX) - HALO) * dh - center_x * dh
+    y = (np.arange(NY) - HALO) * dh - center_y * dh
+    X, Y = np.meshgrid(x, y)
+    lo_x, hi_x = x[HALO + npml], x[HALO + nx - 1 - npml]
+    lo_y, hi_y = y[HALO + npml], y[HALO + ny - 1 - npml]
+    t = np.zeros((NY, NX))
+    for _ in range(nhills):
+        h = rng.uniform(hmin, hmax)
+        s = rng.uniform(smin, smax)
+        cx = rng.uniform(lo_x, hi_x)
+        cy = rng.uniform(lo_y, hi_y)
+        t += h * np.exp(-0.5 * ((X - cx) ** 2 + (Y - cy) ** 2) / s ** 2)
+    return t.astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# media
+# ---------------------------------------------------------------------------
+def homogeneous_medium(shape, vs, vp, rho):
+    m = np.empty((3,) + tuple(shape), np.float32)
+    m[0], m[1], m[2] = vs, vp, rho
+    return m
+
+
+def layered_medium(coord, terrain, tops_m, vs, vp, rho, tolerance=1.0):
+    """medium.cpp:89-213 restated: layer F occupies z < terrain - top_F - tol."""
+    z = coord[2].astype(np.float64)
+    surf = terrain.astype(np.float64)[None, :, :]
+    idx = np.zeros(z.shape, np.int32)
+    for F in range(1, len(tops_m)):
+        idx += (z < surf - tops_m[F] - tolerance).astype(np.int32)
+    m = np.empty((3,) + z.shape, np.float32)
+    m[0] = np.asarray(vs, np.float64)[idx]
+    m[1] = np.asarray(vp, np.float64)[idx]
+    m[2] = np.asarray(rho, np.float64)[idx]
+    return m
+
+
+def gaussian_random_field(shape, dh, corr_len, sigma, seed, clip=3.0):
+    """Zero-mean Gaussian field with Gaussian autocorrelation (FFT-filtered noise)."""
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal(shape).astype(np.float32)
+    spec = np.fft.rfftn(noise)
+    ks = [np.fft.fftfreq(n, d=dh) for n in shape[:-1]] + [np.fft.rfftfreq(shape[-1], d=dh)]
+    filt = np.ones(spec.shape, np.float32)
+    for ax, k in enumerate(ks):
+        sh = [1] * len(shape)
+        sh[ax] = len(k)
+        # autocorrelation exp(-r^2/a^2) <=> spectrum exp(-(pi a k)^2); amplitude = sqrt
+        filt = filt * np.exp(-0.5 * (math.pi * corr_len * k) ** 2).astype(np.float32).reshape(sh)
+    f = np.fft.irfftn(spec * filt, s=shape, axes=tuple(range(len(shape)))).astype(np.float32)
+    f *= sigma / max(float(f.std()), 1e-30)
+    return np.clip(f, -clip * sigma, clip * sigma).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# sources
+# ---------------------------------------------------------------------------
+def moment_tensor(strike_deg, dip_deg, rake_deg):
+    """multiSource.cpp:507-519: (Mxx, Myy, Mzz, Mxy, Mxz, Myz), x = east, y = north, z = up."""
+    d2r = math.pi / 180.0
+    s, d, r = strike_deg * d2r, dip_deg * d2r, rake_deg * d2r
+    M11 = -(math.sin(d) * math.cos(r) * math.sin(2 * s) + math.sin(2 * d) * math.sin(r) * math.sin(s) ** 2)
+    M22 = math.sin(d) * math.cos(r) * math.sin(2 * s) - math.sin(2 * d) * math.sin(r) * math.cos(s) ** 2
+    M33 = -(M11 + M22)
+    M12 = math.sin(d) * math.cos(r) * math.cos(2 * s) + 0.5 * math.sin(2 * d) * math.sin(r) * math.sin(2 * s)
+    M13 = -(math.cos(d) * math.cos(r) * math.cos(s) + math.cos(2 * d) * math.sin(r) * math.sin(s))
+    M23 = -(math.cos(d) * math.cos(r) * math.sin(s) - math.cos(2 * d) * math.sin(r) * math.cos(s))
+    return np.array([M22, M11, M33, M12, -M23, -M13], np.float64)
+
+
+def ricker_rate(fc, t):
+    """singleSource.cpp:42-68 time function divided by its 1e9 amplitude."""
+    r = math.pi * fc * (np.asarray(t, np.float64) - 1.0 / fc)
+    rr = r * r
+    return r * (3.0 - 2.0 * rr) * np.exp(-rr) * math.sqrt(math.pi) * 0.5 * fc * math.pi
+
+
+def triangle_rate(t, t0, rise, slip):
+    """Triangular slip rate of duration `rise` starting at t0, integral = slip."""
+    t = np.asarray(t, np.float64)
+    half = 0.5 * rise
+    peak = 2.0 * slip / rise
+    u = t - t0
+    out = np.where((u >= 0) & (u < half), peak * u / half, 0.0)
+    out = np.where((u >= half) & (u < rise), peak * (rise - u) / half, out)
+    return out
+
+
+def point_source(model: Model, i, j, k, strike, dip, rake, m0, fc, nt_src=None):
+    """One double couple delivered as a one-point moment-rate table (area = 1,
+    slip-rate = M0 g(t) / mu) so it goes through the finite-fault path."""
+    NZ, NY, NX = model.shape_padded
+    mu = float(model.medium[2, k, j, i]) * float(model.medium[0, k, j, i]) ** 2
+    nt_src = nt_src or model.nt + 2
+    t = np.arange(nt_src) * model.dt
+    g = ricker_rate(fc, t)
+    M = moment_tensor(strike, dip, rake)
+    rate = (M[None, :] * (m0 * g / mu)[:, None]).astype(np.float32)
+    model.src_cells = np.array([model.cell_index(i, j, k)], np.int64)
+    model.src_rate = rate[None]
+    model.src_dt = model.dt
+    return model
+
+
+# ---------------------------------------------------------------------------
+# time step from the CFL estimate
+# ---------------------------------------------------------------------------
+def hmin_point_plane(coord):
+    """calc_CFL.cpp:94-163 over the interior: min distance from a point to the
+    plane through its -/+ neighbours in x, y, z (8 combinations)."""
+    H = HALO
+    NZ = coord.shape[1]
+    best = np.inf
+    chunk = 8
+    for k0 in range(H, NZ - H, chunk):
+        k1 = min(k0 + chunk, NZ - H)
+        X = coord[:, k0 - 1:k1 + 1].astype(np.float64)   # one plane of margin each side
+        P = X[:, 1:-1, H:-H, H:-H]
+        for a in (-1, 1):
+            for b in (-1, 1):
+                for c in (-1, 1):
+                    A = np.roll(X, a, axis=3)[:, 1:-1, H:-H, H:-H]
+                    B = np.roll(X, b, axis=2)[:, 1:-1, H:-H, H:-H]
+                    C = np.roll(X, c, axis=1)[:, 1:-1, H:-H, H:-H]
+                    n = np.cross(B - A, C - A, axis=0)
+                    nn = np.sqrt((n * n).sum(0))
+                    d = np.abs(((P - A) * n).sum(0) / nn)
+                    best = min(best, float(d.min()))
+    return best
+
+
+# ---------------------------------------------------------------------------
+# the five benchmark configurations (scaled copies for tests)
+# ---------------------------------------------------------------------------
+def surface_station_grid(nx, ny, nz, npml, n_per_side):
+    I = np.round(np.linspace(npml, nx - 1 - npml, n_per_side)).astype(int) + HALO
+    J = np.round(np.linspace(npml, ny - 1 - npml, n_per_side)).astype(int) + HALO
+    K = nz - 1 + HALO
+    st = np.array([[i, j, K] for j in J for i in I], np.int32)
+    return st
+
+
+def config1(nt=1000, nx=128, ny=128, nz=64, dh=100.0, npml=12):
+    """Plain CGFDM: flat half-space Vp 6000 / Vs 3464 / rho 2700, double couple
+    (30, 60, 90, M0 1e16) at 3 km depth, 2 Hz Ricker-type rate, CFL 0.5."""
+    coord, terrain = flat_coord(nx, ny, nz, dh)
+    med = homogeneous_medium(coord.shape[1:], 3464.0, 6000.0, 2700.0)
+    hmin = dh / math.sqrt(3.0)
+    dt = 0.5 * hmin / 6000.0
+    m = Model(nx, ny, nz, dh, dt, nt, npml, coord, med, True, name="config1", terrain=terrain)
+    depth_cells = int(round(3000.0 / dh))
+    i, j, k = nx // 2 + HALO, ny // 2 + HALO, nz - 1 - depth_cells + HALO
+    point_source(m, i, j, k, 30.0, 60.0, 90.0, 1e16, 2.0)
+    picks = [int(round(f * nx)) for f in (20 / 128, 42 / 128, 64 / 128, 86 / 128, 108 / 128)]
+    m.stations = np.array([[a + HALO, b + HALO, nz - 1 + HALO] for b in picks for a in picks], np.int32)
+    return m
+
+
+def config2(nt=3000, nx=512, ny=512, nz=256, dh=100.0, npml=20, seed=1002, nhills=8,
+            perturb=True):
+    """CGFDM curvilinear: 8 seeded Gaussian hills, 4-layer medium with a 5%
+    Gaussian random perturbation (1 km correlation), 20-layer PML, point
+    moment source at 5 km depth, 400 surface stations, PGV map."""
+    depth = nz * dh
+    terrain = gaussian_hills(nx, ny, dh, npml, nhills, seed)
+    coord = terrain_coord(nx, ny, nz, dh, terrain, depth)
+    vp = np.array([3000.0, 4500.0, 6000.0, 7000.0])
+    med = layered_medium(coord, terrain, [0.0, 2000.0, 8000.0, 18000.0], vp / 1.73, vp,
+                         [2300.0, 2500.0, 2700.0, 3000.0])
+    if perturb:
+        delta = gaussian_random_field(med.shape[1:], dh, 1000.0, 0.05, seed + 7)
+        med[0] *= 1.0 + delta
+        med[1] *= 1.0 + delta
+    hmin = hmin_point_plane(coord)
+    dt = 0.5 * hmin / float(med[1].max())
+    m = Model(nx, ny, nz, dh, dt, nt, npml, coord, med, True, name="config2", terrain=terrain)
+    depth_cells = int(round(5000.0 / dh))
+    i, j, k = nx // 2 + HALO, ny // 2 + HALO, nz - 1 - depth_cells + HALO
+    point_source(m, i, j, k, 30.0, 60.0, 90.0, 1e16, 2.0)
+    nside = 20 if min(nx, ny) - 2 * npml >= 20 else max(2, min(nx, ny) - 2 * npml)
+    m.stations = surface_station_grid(nx, ny, nz, npml, nside)
+    return m
+
+
+def small_config(nx=24, ny=20, nz=16, dh=100.0, npml=4, nt=12, seed=7, hills=2, layered=True,
+                 free_surface=True):
+    """Small curvilinear layered case used by the parity tests."""
+    depth = nz * dh
+    if hills:
+        terrain = gaussian_hills(nx, ny, dh, npml, hills, seed, hmin=100.0, hmax=300.0,
+                                 smin=300.0, smax=900.0)
+    else:
+        terrain = np.zeros((ny + 2 * HALO, nx + 2 * HALO), np.float32)
+    coord = terrain_coord(nx, ny, nz, dh, terrain, depth)
+    if layered:
+        vp = np.array([3000.0, 4500.0, 6000.0])
+        med = layered_medium(coord, terrain, [0.0, 400.0, 900.0], vp / 1.73, vp, [2300.0, 2500.0, 2700.0])
+        delta = gaussian_random_field(med.shape[1:], dh, 300.0, 0.05, seed + 1)
+        med[0] *= 1.0 + delta
+        med[1] *= 1.0 + delta
+    else:
+        med = homogeneous_medium(coord.shape[1:], 3464.0, 6000.0, 2700.0)
+    hmin = hmin_point_plane(coord)
+    dt = 0.5 * hmin / float(med[1].max())
+    m = Model(nx, ny, nz, dh, dt, nt, npml, coord, med, free_surface, name="small", terrain=terrain)
+    i, j, k = nx // 2 + HALO, ny // 2 + HALO, nz // 2 + HALO
+    point_source(m, i, j, k, 30.0, 60.0, 90.0, 1e16, 8.0)
+    m.stations = np.array([[i, j, nz - 1 + HALO], [HALO + 1, HALO + 2, nz - 1 + HALO],
+                           [i + 2, j - 3, k], [nx + HALO - 1, ny + HALO - 1, HALO]], np.int32)
+    return m

@@ -1,0 +1,639 @@
+// C ABI of the MI355X CGFDM library: solver lifetime, setup, time stepping
+// and observers. Host orchestration only; all grid work runs in the HIP
+// kernels of setup.hip and cgfdm_step.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cgfdm_amd.h"
+#include "common.h"
+#include "kernels.h"
+#include "setup.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) return fail(CGFDM_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// F/B schedule of the MacCormack operators, selected by it % 8 and flipped
+// after every RK stage (reference propagate.cpp:158-168, 195-197, 248-250).
+const int FBTAB[8][3] = {{-1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, -1, 1},
+                         {-1, 1, -1},  {1, -1, -1}, {1, -1, 1}, {-1, 1, 1}};
+
+constexpr double REF_PI = 3.141592657;  // reference macro.h:70, enters the PML alpha profile
+
+struct SolverBase {
+  virtual ~SolverBase() {}
+  virtual int init(const cgfdm_desc& d, const float* coord, const float* medium) = 0;
+  virtual int set_source(int np, int nt, double dt, const long long* cell, const float* rate) = 0;
+  virtual int set_stations(int ns, const int* ijk) = 0;
+  virtual int step(int n) = 0;
+  virtual int get_stations(float* out) = 0;
+  virtual int get_pgv(float* out) = 0;
+  virtual int get_dis(float* out) = 0;
+  virtual int get_field(int comp, double* out) = 0;
+  virtual int set_field(int comp, const double* in) = 0;
+  virtual int get_setup(int what, double* out) = 0;
+  virtual int timing(int enable, double* out) = 0;
+  hipStream_t stream = nullptr;
+  int it = 0;
+  double ranges[8] = {0};
+  long long bytes = 0;
+};
+
+template <class T>
+float to_float(T v) { return (float)v; }
+template <>
+float to_float<__half>(__half v) { return __half2float(v); }
+template <>
+float to_float<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
+
+template <class R, class TM>
+struct Solver : SolverBase {
+  cgfdm_desc d{};
+  int NX = 0, NY = 0, NZ = 0;
+  long long N = 0, S = 0;
+  double Z = 1.0;  // stress carried as T / Z
+  std::vector<void*> allocs;
+  R *A = nullptr, *B = nullptr, *C = nullptr, *T = nullptr;
+  TM* coef = nullptr;
+  R* jac = nullptr;
+  R *rdzdx = nullptr, *rdzdy = nullptr;
+  R* pml_par = nullptr;  // alpha, beta, d for x (NX each), then y, then z
+  cg::PmlArgs<R> pml{};
+  float* pgv = nullptr;
+  R* dis = nullptr;
+  int ns = 0;
+  long long* st_cell = nullptr;
+  float* st_rec = nullptr;
+  int src_np = 0, src_nt = 0;
+  double src_dt = 0;
+  long long* src_cell = nullptr;
+  R* src_rate = nullptr;
+  int kchunk = 64;
+  // timing
+  bool timing_on = false;
+  struct EvPair {
+    hipEvent_t a, b;
+    int kind;  // 0 interior kernel, 1 surface kernel, 2 whole step
+  };
+  std::vector<EvPair> ev;
+  size_t ev_used = 0;
+  double tacc[6] = {0};
+
+  ~Solver() override {
+    for (auto& e : ev) {
+      hipEventDestroy(e.a);
+      hipEventDestroy(e.b);
+    }
+    for (void* p : allocs) hipFree(p);
+  }
+
+  template <class X>
+  int dalloc(X** p, long long count) {
+    size_t nb = (size_t)count * sizeof(X);
+    if (nb == 0) nb = 16;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, nb);
+    if (e != hipSuccess) return fail(CGFDM_ERR_ALLOC, "hipMalloc of " + std::to_string(nb) + " bytes failed");
+    e = hipMemsetAsync(q, 0, nb, stream);
+    if (e != hipSuccess) return fail(CGFDM_ERR_HIP, "hipMemset failed");
+    allocs.push_back(q);
+    bytes += (long long)nb;
+    *p = (X*)q;
+    return CGFDM_OK;
+  }
+  void dfree(void* p) {
+    for (size_t q = 0; q < allocs.size(); ++q)
+      if (allocs[q] == p) {
+        hipFree(p);
+        allocs.erase(allocs.begin() + q);
+        return;
+      }
+  }
+
+  int init(const cgfdm_desc& dd, const float* coord, const float* medium) override {
+    d = dd;
+    NX = d.nx + 2 * cg::H;
+    NY = d.ny + 2 * cg::H;
+    NZ = d.nz + 2 * cg::H;
+    S = (long long)NX * NY;
+    N = S * NZ;
+    if (const char* e = std::getenv("CGFDM_KCHUNK")) kchunk = std::max(4, std::atoi(e));
+    int rc;
+    // --- temporaries: coordinates, medium, FP64 metrics
+    float *dcoord = nullptr, *dmed = nullptr;
+    double* met = nullptr;
+    if ((rc = dalloc(&dcoord, 3 * N))) return rc;
+    if ((rc = dalloc(&dmed, 3 * N))) return rc;
+    if ((rc = dalloc(&met, 10 * N))) return rc;
+    HIPCHK(hipMemcpyAsync(dcoord, coord, sizeof(float) * 3 * N, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dmed, medium, sizeof(float) * 3 * N, hipMemcpyHostToDevice, stream));
+    cg::launch_metrics(dcoord, met, d.nx, d.ny, d.nz, NX, NY, N, d.dh, d.free_surface != 0, stream);
+    HIPCHK(hipGetLastError());
+
+    // --- CFL ranges (calc_CFL.cpp:255-349)
+    {
+      const int nb = 1024;
+      double* part = nullptr;
+      if ((rc = dalloc(&part, 2 * nb * 4))) return rc;
+      cg::launch_hrange(dcoord, part, nb, d.nx, d.ny, d.nz, NX, NY, N, stream);
+      for (int c = 0; c < 3; ++c) cg::launch_frange(dmed + c * N, N, part + 2 * nb * (c + 1), nb, stream);
+      HIPCHK(hipGetLastError());
+      std::vector<double> h(2 * nb * 4);
+      HIPCHK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      for (int c = 0; c < 4; ++c) {
+        double lo = 1e300, hi = -1e300;
+        for (int b = 0; b < nb; ++b) {
+          lo = std::min(lo, h[2 * nb * c + 2 * b]);
+          hi = std::max(hi, h[2 * nb * c + 2 * b + 1]);
+        }
+        ranges[2 * c] = lo;
+        ranges[2 * c + 1] = hi;
+      }
+      dfree(part);
+      // ranges = [hmin, hmax, vsmin, vsmax, vpmin, vpmax, rhomin, rhomax]
+      double dtmax = 1.34 * ranges[0] / ranges[5];
+      if (d.dt > dtmax)
+        return fail(CGFDM_ERR_CFL, "DT " + std::to_string(d.dt) + " exceeds the CFL limit " + std::to_string(dtmax));
+    }
+    if (d.coef_storage != 0) Z = std::sqrt(ranges[6] * ranges[7]) * std::sqrt(ranges[4] * ranges[5]);
+
+    // --- packed coefficients
+    if ((rc = dalloc(&coef, (long long)cg::M_NCOEF * N))) return rc;
+    if ((rc = dalloc(&jac, N))) return rc;
+    cg::launch_pack<R, TM>(met, dmed, coef, jac, N, Z, stream);
+    if ((rc = dalloc(&rdzdx, 9 * S))) return rc;
+    if ((rc = dalloc(&rdzdy, 9 * S))) return rc;
+    if (d.free_surface) cg::launch_rdz<R>(met, dmed, rdzdx, rdzdy, d.nx, d.ny, d.nz, NX, NY, N, stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
+    dfree(met);
+    dfree(dmed);
+    dfree(dcoord);
+
+    // --- PML profiles (init_pml_para.cpp:97-239), host FP64, then upload
+    {
+      int nn[3] = {NX, NY, NZ};
+      std::vector<R> hp(3 * (size_t)(NX + NY + NZ));
+      double L = d.dh * (d.npml - 1);
+      double Rc = d.npml > 0 ? std::pow(10.0, -(std::log10((double)d.npml) - 1.0) / std::log10(2.0) - 3.0) : 1.0;
+      double d0 = d.npml > 1 ? -(2.0 + 1.0) * 6000.0 / (2.0 * L) * std::log(Rc) : 0.0;
+      double a0 = REF_PI * 0.5, b0 = 3.0;
+      size_t off = 0;
+      for (int ax = 0; ax < 3; ++ax) {
+        int n = nn[ax];
+        R* pa = hp.data() + off;
+        R* pb = pa + n;
+        R* pd = pb + n;
+        for (int q = 0; q < n; ++q) {
+          pa[q] = R(0);
+          pb[q] = R(1);
+          pd[q] = R(0);
+        }
+        int lo = cg::H, hi = n - cg::H;
+        for (int q = lo; q < hi; ++q) {
+          double l = -1.0;
+          if (d.npml > 0) {
+            if (q < cg::H + d.npml) l = (cg::H + d.npml - 1 - q) * d.dh;
+            if (q >= hi - d.npml && !(ax == 2 && d.free_surface)) l = (q - (hi - d.npml)) * d.dh;
+          }
+          double al = 0, be = 1, dd = 0;
+          if (l >= 0 && L > 0) {
+            al = a0 * (1.0 - l / L);
+            be = 1.0 + (b0 - 1.0) * (l / L) * (l / L);
+            dd = d0 * (l / L) * (l / L);
+          } else if (l >= 0) {
+            al = a0;
+          }
+          pa[q] = R(al);
+          pd[q] = R(dd / be);
+          pb[q] = R(1.0 / be);
+        }
+        off += 3 * (size_t)n;
+      }
+      if ((rc = dalloc(&pml_par, (long long)hp.size()))) return rc;
+      HIPCHK(hipMemcpyAsync(pml_par, hp.data(), sizeof(R) * hp.size(), hipMemcpyHostToDevice, stream));
+      off = 0;
+      for (int ax = 0; ax < 3; ++ax) {
+        pml.alpha[ax] = pml_par + off;
+        pml.beta[ax] = pml_par + off + nn[ax];
+        pml.d[ax] = pml_par + off + 2 * nn[ax];
+        off += 3 * (size_t)nn[ax];
+      }
+      pml.npml = d.npml;
+      pml.has_z2 = (d.npml > 0 && !d.free_surface) ? 1 : 0;
+      long long cnt[6] = {(long long)d.npml * d.ny * d.nz, (long long)d.nx * d.npml * d.nz,
+                          (long long)d.nx * d.ny * d.npml, (long long)d.npml * d.ny * d.nz,
+                          (long long)d.nx * d.npml * d.nz, (long long)d.nx * d.ny * d.npml};
+      for (int f = 0; f < 6; ++f) {
+        pml.f[f] = cg::FaceBuf<R>{nullptr, nullptr, nullptr, 0};
+        if (d.npml <= 0 || (f == 5 && !pml.has_z2)) continue;
+        R* base = nullptr;
+        if ((rc = dalloc(&base, 27 * cnt[f]))) return rc;
+        pml.f[f] = cg::FaceBuf<R>{base, base + 9 * cnt[f], base + 18 * cnt[f], cnt[f]};
+      }
+    }
+    // --- wavefield buffers and observers
+    if ((rc = dalloc(&A, 9 * N))) return rc;
+    if ((rc = dalloc(&B, 9 * N))) return rc;
+    if ((rc = dalloc(&C, 9 * N))) return rc;
+    if ((rc = dalloc(&T, 9 * N))) return rc;
+    if ((rc = dalloc(&pgv, 2LL * d.nx * d.ny))) return rc;
+    if ((rc = dalloc(&dis, 3 * S))) return rc;
+    HIPCHK(hipStreamSynchronize(stream));
+    return CGFDM_OK;
+  }
+
+  int set_source(int np, int nt, double dt, const long long* cell, const float* rate) override {
+    if (np < 0 || (np > 0 && (nt < 2 || dt <= 0 || !cell || !rate))) return fail(CGFDM_ERR_SOURCE, "bad source table");
+    std::map<long long, std::vector<double>> acc;
+    for (int p = 0; p < np; ++p) {
+      long long c = cell[p];
+      int i = (int)(c % NX), j = (int)((c / NX) % NY), k = (int)(c / S);
+      if (c < 0 || c >= N || i < cg::H || i >= NX - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H ||
+          k >= NZ - cg::H)
+        return fail(CGFDM_ERR_SOURCE, "source cell outside the interior");
+      auto& v = acc[c];
+      if (v.empty()) v.assign((size_t)nt * 6, 0.0);
+      const float* r = rate + (size_t)p * nt * 6;
+      for (size_t q = 0; q < (size_t)nt * 6; ++q) v[q] += r[q];
+    }
+    int rc;
+    if (src_cell) dfree(src_cell);
+    if (src_rate) dfree(src_rate);
+    src_cell = nullptr;
+    src_rate = nullptr;
+    src_np = (int)acc.size();
+    src_nt = nt;
+    src_dt = dt;
+    if (src_np == 0) return CGFDM_OK;
+    std::vector<long long> hc;
+    std::vector<R> hr((size_t)src_np * nt * 6);
+    int Jp = 0;
+    for (auto& kv : acc) {
+      long long c = kv.first;
+      hc.push_back(c);
+      R jv;
+      TM mv;
+      HIPCHK(hipMemcpy(&jv, jac + c, sizeof(R), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(&mv, coef + (long long)cg::M_MU * N + c, sizeof(TM), hipMemcpyDeviceToHost));
+      double mu = (double)to_float(mv);
+      if (sizeof(TM) == 8) mu = *(double*)(void*)&mv;
+      double J = (double)jv;
+      double sc = mu / (J * d.dh * d.dh * d.dh);  // mu/(J DH^3), stress in units of Z
+      for (int t = 0; t < nt; ++t)
+        for (int q = 0; q < 6; ++q) hr[((size_t)t * src_np + Jp) * 6 + q] = R(kv.second[(size_t)t * 6 + q] * sc);
+      ++Jp;
+    }
+    if ((rc = dalloc(&src_cell, src_np))) return rc;
+    if ((rc = dalloc(&src_rate, (long long)hr.size()))) return rc;
+    HIPCHK(hipMemcpy(src_cell, hc.data(), sizeof(long long) * src_np, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(src_rate, hr.data(), sizeof(R) * hr.size(), hipMemcpyHostToDevice));
+    return CGFDM_OK;
+  }
+
+  int set_stations(int n, const int* ijk) override {
+    if (n < 0 || (n > 0 && !ijk)) return fail(CGFDM_ERR_ARG, "bad station list");
+    std::vector<long long> hc(n);
+    for (int s = 0; s < n; ++s) {
+      int i = ijk[3 * s], j = ijk[3 * s + 1], k = ijk[3 * s + 2];
+      if (i < cg::H || i >= NX - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H || k >= NZ - cg::H)
+        return fail(CGFDM_ERR_ARG, "station outside the interior");
+      hc[s] = (long long)i + (long long)NX * j + (long long)k * S;
+    }
+    int rc;
+    if (st_cell) dfree(st_cell);
+    if (st_rec) dfree(st_rec);
+    st_cell = nullptr;
+    st_rec = nullptr;
+    ns = n;
+    if (n == 0) return CGFDM_OK;
+    if ((rc = dalloc(&st_cell, n))) return rc;
+    if ((rc = dalloc(&st_rec, (long long)n * d.nt * 9))) return rc;
+    HIPCHK(hipMemcpy(st_cell, hc.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    return CGFDM_OK;
+  }
+
+  size_t next_pair(int kind) {
+    if (ev_used >= ev.size()) {
+      EvPair e{};
+      hipEventCreate(&e.a);
+      hipEventCreate(&e.b);
+      ev.push_back(e);
+    }
+    ev[ev_used].kind = kind;
+    return ev_used++;
+  }
+
+  int step(int n) override {
+    const int kend = cg::H + d.nz - (d.free_surface ? 3 : 0);
+    for (int s = 0; s < n; ++s) {
+      int fb[3] = {FBTAB[it % 8][0], FBTAB[it % 8][1], FBTAB[it % 8][2]};
+      size_t sp = 0;
+      if (timing_on) {
+        sp = next_pair(2);
+        hipEventRecord(ev[sp].a, stream);
+      }
+      // moment-rate source on W at t = it * DT (addMoment.cpp:262-377), float
+      // time arithmetic as in the reference
+      if (src_np > 0) {
+        float DTf = (float)d.dt;
+        float tmpT = ((float)it + 0.0f) * DTf;
+        int srcIt = int(tmpT / (float)src_dt);
+        if (srcIt + 1 < src_nt) {
+          float t1 = float(srcIt) * (float)src_dt;
+          float tw = (tmpT - t1) / (float)src_dt;
+          cg::launch_source<R>(A, N, src_np, src_cell, src_rate, srcIt, R(tw), R(d.dt), stream);
+        }
+      }
+      for (int irk = 0; irk < 4; ++irk) {
+        cg::StageArgs<R> a{};
+        a.nx = d.nx;
+        a.ny = d.ny;
+        a.nz = d.nz;
+        a.NX = NX;
+        a.NY = NY;
+        a.N = N;
+        a.S = S;
+        a.kbeg = cg::H;
+        a.kend = kend;
+        a.kc = kchunk;
+        a.stage = irk;
+        a.DT = R(d.dt);
+        a.rDH = R(1.0 / d.dh);
+        a.win = irk == 0 ? A : irk == 1 ? B : irk == 2 ? C : B;
+        a.wout = irk == 0 ? B : irk == 1 ? C : irk == 2 ? B : A;
+        a.wm = A;
+        a.wt = T;
+        a.pml = pml;
+        size_t ip = 0;
+        if (timing_on) {
+          ip = next_pair(0);
+          hipEventRecord(ev[ip].a, stream);
+        }
+        cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream);
+        if (timing_on) hipEventRecord(ev[ip].b, stream);
+        if (d.free_surface) {
+          cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, pgv, dis};
+          size_t fp = 0;
+          if (timing_on) {
+            fp = next_pair(1);
+            hipEventRecord(ev[fp].a, stream);
+          }
+          cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], stream);
+          if (timing_on) hipEventRecord(ev[fp].b, stream);
+        }
+        fb[0] = -fb[0];
+        fb[1] = -fb[1];
+        fb[2] = -fb[2];
+      }
+      cg::launch_record<R>(A, N, ns, st_cell, st_rec, d.nt, it, stream);
+      if (timing_on) hipEventRecord(ev[sp].b, stream);
+      ++it;
+    }
+    HIPCHK(hipGetLastError());
+    return CGFDM_OK;
+  }
+
+  int timing(int enable, double* out) override {
+    if (timing_on && !enable) {
+      HIPCHK(hipStreamSynchronize(stream));
+      for (size_t q = 0; q < ev_used; ++q) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[q].a, ev[q].b));
+        tacc[2 * ev[q].kind] += ms;
+        tacc[2 * ev[q].kind + 1] += 1;
+      }
+      ev_used = 0;
+      timing_on = false;
+    } else if (enable) {
+      for (double& t : tacc) t = 0;
+      ev_used = 0;
+      timing_on = true;
+    }
+    if (out) std::memcpy(out, tacc, sizeof(tacc));
+    return CGFDM_OK;
+  }
+
+  int get_stations(float* out) override {
+    if (ns == 0) return CGFDM_OK;
+    HIPCHK(hipStreamSynchronize(stream));
+    size_t n = (size_t)ns * d.nt * 9;
+    HIPCHK(hipMemcpy(out, st_rec, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (Z != 1.0)
+      for (size_t q = 0; q < n; ++q)
+        if (q % 9 >= 3) out[q] = (float)(out[q] * Z);
+    return CGFDM_OK;
+  }
+
+  int get_pgv(float* out) override {
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(out, pgv, sizeof(float) * 2 * (size_t)d.nx * d.ny, hipMemcpyDeviceToHost));
+    return CGFDM_OK;
+  }
+
+  int get_dis(float* out) override {
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<R> h(3 * (size_t)S);
+    HIPCHK(hipMemcpy(h.data(), dis, sizeof(R) * h.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < 3; ++c)
+      for (int j = 0; j < d.ny; ++j)
+        for (int i = 0; i < d.nx; ++i)
+          out[((size_t)c * d.ny + j) * d.nx + i] = (float)h[(size_t)c * S + (size_t)(j + cg::H) * NX + (i + cg::H)];
+    return CGFDM_OK;
+  }
+
+  int get_field(int comp, double* out) override {
+    if (comp < 0 || comp > 8) return fail(CGFDM_ERR_ARG, "component out of range");
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<R> h(N);
+    HIPCHK(hipMemcpy(h.data(), A + (size_t)comp * N, sizeof(R) * N, hipMemcpyDeviceToHost));
+    double sc = comp >= 3 ? Z : 1.0;
+    for (long long p = 0; p < N; ++p) out[p] = (double)h[p] * sc;
+    return CGFDM_OK;
+  }
+
+  int set_field(int comp, const double* in) override {
+    if (comp < 0 || comp > 8) return fail(CGFDM_ERR_ARG, "component out of range");
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<R> h(N);
+    double sc = comp >= 3 ? 1.0 / Z : 1.0;
+    for (long long p = 0; p < N; ++p) h[p] = R(in[p] * sc);
+    HIPCHK(hipMemcpy(A + (size_t)comp * N, h.data(), sizeof(R) * N, hipMemcpyHostToDevice));
+    return CGFDM_OK;
+  }
+
+  int get_setup(int what, double* out) override {
+    HIPCHK(hipStreamSynchronize(stream));
+    if (what >= 0 && what <= 12 && what != 9) {
+      int plane = what < 9 ? what : what - 1;  // 10,11,12 -> M_MU.. (9,10,11)
+      std::vector<TM> h(N);
+      HIPCHK(hipMemcpy(h.data(), coef + (size_t)plane * N, sizeof(TM) * N, hipMemcpyDeviceToHost));
+      double sc = what == 10 || what == 11 ? Z : what == 12 ? 1.0 / Z : 1.0;
+      for (long long p = 0; p < N; ++p) {
+        double v = sizeof(TM) == 8 ? (double)*(const double*)(const void*)&h[p] : (double)to_float(h[p]);
+        out[p] = v * sc;
+      }
+      return CGFDM_OK;
+    }
+    if (what == 9) {
+      std::vector<R> h(N);
+      HIPCHK(hipMemcpy(h.data(), jac, sizeof(R) * N, hipMemcpyDeviceToHost));
+      for (long long p = 0; p < N; ++p) out[p] = (double)h[p];
+      return CGFDM_OK;
+    }
+    if ((what >= 20 && what < 29) || (what >= 30 && what < 39)) {
+      const R* src = (what < 30 ? rdzdx : rdzdy) + (size_t)(what % 10) * S;
+      std::vector<R> h(S);
+      HIPCHK(hipMemcpy(h.data(), src, sizeof(R) * S, hipMemcpyDeviceToHost));
+      for (long long p = 0; p < S; ++p) out[p] = (double)h[p];
+      return CGFDM_OK;
+    }
+    if (what >= 40 && what < 49) {
+      int ax = (what - 40) / 3, kind = (what - 40) % 3;
+      int n = ax == 0 ? NX : ax == 1 ? NY : NZ;
+      const R* src = kind == 0 ? pml.alpha[ax] : kind == 1 ? pml.beta[ax] : pml.d[ax];
+      std::vector<R> h(n);
+      HIPCHK(hipMemcpy(h.data(), src, sizeof(R) * n, hipMemcpyDeviceToHost));
+      for (int p = 0; p < n; ++p) out[p] = (double)h[p];
+      return CGFDM_OK;
+    }
+    return fail(CGFDM_ERR_ARG, "unknown setup product");
+  }
+};
+
+}  // namespace
+
+struct cgfdm_solver {
+  std::unique_ptr<SolverBase> impl;
+};
+
+extern "C" {
+
+const char* cgfdm_last_error(void) { return g_err.c_str(); }
+
+int cgfdm_create(const cgfdm_desc* desc, const float* coord, const float* medium, cgfdm_solver** out) {
+  if (!desc || !coord || !medium || !out) return fail(CGFDM_ERR_ARG, "null argument");
+  *out = nullptr;
+  const cgfdm_desc& d = *desc;
+  if (d.nx < 1 || d.ny < 1 || d.nz < 4 || d.dh <= 0 || d.dt <= 0 || d.nt < 0)
+    return fail(CGFDM_ERR_ARG, "bad grid description");
+  if (d.npml < 0) return fail(CGFDM_ERR_ARG, "negative npml");
+  if (d.npml > 0 && (d.npml >= d.nx || d.npml >= d.ny || d.npml >= d.nz))
+    return fail(CGFDM_ERR_PML, "nPML must be smaller than nx, ny and nz");
+  if (d.scheme != CGFDM_SCHEME_CGFDM) return fail(CGFDM_ERR_ARG, "scheme not available in this build");
+  std::unique_ptr<SolverBase> s;
+  if (d.precision == 8) {
+    if (d.coef_storage != 0) return fail(CGFDM_ERR_ARG, "16-bit coefficient storage needs the FP32 wavefield");
+    s.reset(new Solver<double, double>());
+  } else if (d.precision == 4) {
+    if (d.coef_storage == 0) s.reset(new Solver<float, float>());
+    else if (d.coef_storage == 1) s.reset(new Solver<float, __half>());
+    else if (d.coef_storage == 2) s.reset(new Solver<float, __hip_bfloat16>());
+    else return fail(CGFDM_ERR_ARG, "bad coef_storage");
+  } else {
+    return fail(CGFDM_ERR_ARG, "precision must be 4 or 8");
+  }
+  int rc = s->init(d, coord, medium);
+  if (rc) return rc;
+  auto* h = new cgfdm_solver();
+  h->impl = std::move(s);
+  *out = h;
+  return CGFDM_OK;
+}
+
+int cgfdm_destroy(cgfdm_solver* s) {
+  if (!s) return CGFDM_OK;
+  if (s->impl && s->impl->stream) hipStreamSynchronize(s->impl->stream);
+  delete s;
+  return CGFDM_OK;
+}
+
+#define SOLVER_OR_FAIL(s) \
+  if (!(s) || !(s)->impl) return fail(CGFDM_ERR_ARG, "null solver")
+
+int cgfdm_set_source(cgfdm_solver* s, int npts, int nt, double dt, const long long* cell, const float* rate) {
+  SOLVER_OR_FAIL(s);
+  return s->impl->set_source(npts, nt, dt, cell, rate);
+}
+
+int cgfdm_set_stations(cgfdm_solver* s, int ns, const int* ijk) {
+  SOLVER_OR_FAIL(s);
+  return s->impl->set_stations(ns, ijk);
+}
+
+int cgfdm_step(cgfdm_solver* s, int nsteps) {
+  SOLVER_OR_FAIL(s);
+  if (nsteps < 0) return fail(CGFDM_ERR_ARG, "negative step count");
+  return s->impl->step(nsteps);
+}
+
+int cgfdm_set_stream(cgfdm_solver* s, void* stream) {
+  SOLVER_OR_FAIL(s);
+  s->impl->stream = (hipStream_t)stream;
+  return CGFDM_OK;
+}
+
+int cgfdm_synchronize(cgfdm_solver* s) {
+  SOLVER_OR_FAIL(s);
+  hipError_t e = hipStreamSynchronize(s->impl->stream);
+  if (e != hipSuccess) return fail(CGFDM_ERR_HIP, hipGetErrorString(e));
+  return CGFDM_OK;
+}
+
+int cgfdm_step_index(cgfdm_solver* s) { return (s && s->impl) ? s->impl->it : -1; }
+
+int cgfdm_get_stations(cgfdm_solver* s, float* out) {
+  SOLVER_OR_FAIL(s);
+  return s->impl->get_stations(out);
+}
+int cgfdm_get_pgv(cgfdm_solver* s, float* out) {
+  SOLVER_OR_FAIL(s);
+  return s->impl->get_pgv(out);
+}
+int cgfdm_get_surface_displacement(cgfdm_solver* s, float* out) {
+  SOLVER_OR_FAIL(s);
+  return s->impl->get_dis(out);
+}
+int cgfdm_get_field(cgfdm_solver* s, int comp, double* out) {
+  SOLVER_OR_FAIL(s);
+  return s->impl->get_field(comp, out);
+}
+int cgfdm_set_field(cgfdm_solver* s, int comp, const double* in) {
+  SOLVER_OR_FAIL(s);
+  return s->impl->set_field(comp, in);
+}
+int cgfdm_get_setup(cgfdm_solver* s, int what, double* out) {
+  SOLVER_OR_FAIL(s);
+  return s->impl->get_setup(what, out);
+}
+int cgfdm_get_ranges(cgfdm_solver* s, double* out8) {
+  SOLVER_OR_FAIL(s);
+  std::memcpy(out8, s->impl->ranges, sizeof(double) * 8);
+  return CGFDM_OK;
+}
+int cgfdm_timing(cgfdm_solver* s, int enable, double* out) {
+  SOLVER_OR_FAIL(s);
+  return s->impl->timing(enable, out);
+}
+long long cgfdm_device_bytes(cgfdm_solver* s) { return (s && s->impl) ? s->impl->bytes : 0; }
+
+}  // extern "C"

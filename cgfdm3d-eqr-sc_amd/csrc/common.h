@@ -1,0 +1,61 @@
+// Shared definitions of the MI355X CGFDM library (device and host side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <cstdint>
+
+namespace cg {
+
+constexpr int H = 3;  // ghost layers on every side of the local domain (reference macro.h:39)
+
+// MacCormack one-sided coefficients (reference macro.h:121-125) and the 3-point
+// one-sided stencil one layer under the surface (macro.h:144-146).
+constexpr double AF_1 = -0.30874, AF0 = -0.6326, AF1 = 1.2330, AF2 = -0.3334, AF3 = 0.04168;
+constexpr double CF1 = -1.16666667, CF2 = 1.33333333, CF3 = -0.16666667;
+// Classical RK4 written on (W, t, m) buffers (macro.h:133-142).
+constexpr double RK_A2 = 0.5, RK_A3 = 0.5;
+constexpr double RK_B1 = 0.16666667, RK_B2 = 0.33333333, RK_B3 = 0.33333333, RK_B4 = 0.16666667;
+
+// Storage type of metric and medium coefficients.
+template <class T> struct Store;
+template <> struct Store<float> {
+  static __device__ __forceinline__ float ld(const float* p) { return *p; }
+};
+template <> struct Store<double> {
+  static __device__ __forceinline__ double ld(const double* p) { return *p; }
+};
+template <> struct Store<__half> {
+  static __device__ __forceinline__ float ld(const __half* p) { return __half2float(*p); }
+};
+template <> struct Store<__hip_bfloat16> {
+  static __device__ __forceinline__ float ld(const __hip_bfloat16* p) { return __bfloat162float(*p); }
+};
+
+// Coefficient planes, SoA over the padded grid. Metrics are stored as the
+// unit-free products m * DH (i.e. divided by rDH) only when `scaled` is set;
+// the default keeps the reference meaning (xi_x ... zt_z in 1/m).
+enum { M_XIX = 0, M_XIY, M_XIZ, M_ETX, M_ETY, M_ETZ, M_ZTX, M_ZTY, M_ZTZ, M_MU, M_LAM, M_B, M_NCOEF };
+
+// One PML face slab (reference pml_deriv.cpp:17-37): three RK buffers of 9 SoA
+// components over `cnt` cells. a = state at the step start (and the final
+// state), b = intermediate stage state (updated in place), t = accumulator.
+template <class R>
+struct FaceBuf {
+  R* a;
+  R* b;
+  R* t;
+  long long cnt;
+};
+
+template <class R>
+struct PmlArgs {
+  int npml;
+  int has_z2;         // bottom-only PML in z when a free surface is present
+  const R* alpha[3];  // per axis, padded length
+  const R* beta[3];   // inverted beta (1 outside the PML)
+  const R* d[3];      // d / beta
+  FaceBuf<R> f[6];    // x1, y1, z1, x2, y2, z2
+};
+
+}  // namespace cg

@@ -1,0 +1,43 @@
+// Kernel argument blocks and launcher declarations of the CGFDM step.
+#pragma once
+#include "common.h"
+
+namespace cg {
+
+template <class R>
+struct StageArgs {
+  int nx, ny, nz;       // interior cells
+  int NX, NY;           // padded x/y extents
+  long long N;          // padded cells per field
+  long long S;          // NX * NY
+  int kbeg, kend;       // k range of the interior kernel [kbeg, kend)
+  int kc;               // k chunk per block
+  int ntx, nty, nkc, ntiles;  // filled by the launcher
+  int stage;            // RK4 stage 0..3
+  R DT, rDH;
+  const R* win;         // stencil input (9 SoA fields)
+  const R* wm;          // RK base (state at the start of the step)
+  R* wt;                // RK accumulator
+  R* wout;              // stage output
+  PmlArgs<R> pml;
+};
+
+template <class R>
+struct SurfArgs {
+  const R* jac;         // Jacobian, padded grid
+  const R* rdzdx;       // 9 planes [NY][NX]: -DZ^-1 DX on the surface
+  const R* rdzdy;       // 9 planes [NY][NX]: -DZ^-1 DY
+  float* pgv;           // [ny][nx][2] or nullptr (updated on stage 3)
+  R* dis;               // surface displacement, 3 planes [NY][NX]
+};
+
+template <class R, class TM>
+void launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, hipStream_t st);
+template <class R, class TM>
+void launch_surface(StageArgs<R> a, const TM* coef, SurfArgs<R> s, int fb1, int fb2, int fb3, hipStream_t st);
+template <class R>
+void launch_source(R* W, long long N, int np, const long long* cell, const R* rate, int srcIt, R tw, R DT, hipStream_t st);
+template <class R>
+void launch_record(const R* W, long long N, int ns, const long long* cell, float* rec, int nt, int it, hipStream_t st);
+
+}  // namespace cg

@@ -1,0 +1,19 @@
+// Setup-kernel launchers (FP64 metrics, packing, surface matrices, CFL ranges).
+#pragma once
+#include "common.h"
+
+namespace cg {
+
+void launch_metrics(const float* coord, double* met, int nx, int ny, int nz, int NX, int NY, long long N, double dh,
+                    bool fs, hipStream_t st);
+template <class R, class TM>
+void launch_pack(const double* met, const float* med, TM* coef, R* jac, long long N, double stress_scale,
+                 hipStream_t st);
+template <class R>
+void launch_rdz(const double* met, const float* med, R* rdzdx, R* rdzdy, int nx, int ny, int nz, int NX, int NY,
+                long long N, hipStream_t st);
+void launch_hrange(const float* coord, double* part, int nblocks, int nx, int ny, int nz, int NX, int NY, long long N,
+                   hipStream_t st);
+void launch_frange(const float* a, long long n, double* part, int nblocks, hipStream_t st);
+
+}  // namespace cg

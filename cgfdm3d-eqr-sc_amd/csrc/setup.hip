@@ -1,0 +1,276 @@
+// Setup kernels: curvilinear metrics, Jacobian, free-surface matrices,
+// coefficient packing and the CFL grid-spacing reduction, all in FP64 on the
+// GPU (the stored coefficients are then rounded to the storage type).
+#include "common.h"
+#include "setup.h"
+
+namespace cg {
+
+__device__ __forceinline__ double lc(const float* c, long long p, long long s, int fb) {
+  long long d = fb * s;
+  return fb * (AF_1 * (double)c[p - d] + AF0 * (double)c[p] + AF1 * (double)c[p + d] + AF2 * (double)c[p + 2 * d] +
+               AF3 * (double)c[p + 3 * d]);
+}
+
+// Covariant derivatives as the mean of the forward and backward MacCormack
+// differences of the coordinates, J = det, contravariant = cofactor / J
+// (reference contravariant.cpp:182-285). Output: 10 planes (xi_x..zt_z, J).
+__global__ void k_metrics(const float* __restrict__ coord, double* __restrict__ met, int nx, int ny, int nz, int NX,
+                          int NY, long long N, double rdh) {
+  int i = H + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = H + blockIdx.y;
+  int k = H + blockIdx.z;
+  if (i >= H + nx) return;
+  const long long S = (long long)NX * NY;
+  long long p = (long long)i + (long long)NX * j + (long long)k * S;
+  const float* X = coord;
+  const float* Y = coord + N;
+  const float* Z = coord + 2 * N;
+  double x_xi = 0.5 * (lc(X, p, 1, 1) + lc(X, p, 1, -1)) * rdh;
+  double x_et = 0.5 * (lc(X, p, NX, 1) + lc(X, p, NX, -1)) * rdh;
+  double x_zt = 0.5 * (lc(X, p, S, 1) + lc(X, p, S, -1)) * rdh;
+  double y_xi = 0.5 * (lc(Y, p, 1, 1) + lc(Y, p, 1, -1)) * rdh;
+  double y_et = 0.5 * (lc(Y, p, NX, 1) + lc(Y, p, NX, -1)) * rdh;
+  double y_zt = 0.5 * (lc(Y, p, S, 1) + lc(Y, p, S, -1)) * rdh;
+  double z_xi = 0.5 * (lc(Z, p, 1, 1) + lc(Z, p, 1, -1)) * rdh;
+  double z_et = 0.5 * (lc(Z, p, NX, 1) + lc(Z, p, NX, -1)) * rdh;
+  double z_zt = 0.5 * (lc(Z, p, S, 1) + lc(Z, p, S, -1)) * rdh;
+  double J = x_xi * y_et * z_zt + x_et * y_zt * z_xi + x_zt * y_xi * z_et - x_zt * y_et * z_xi - x_et * y_xi * z_zt -
+             x_xi * z_et * y_zt;
+  double r = 1.0 / J;
+  met[0 * N + p] = (y_et * z_zt - y_zt * z_et) * r;
+  met[1 * N + p] = (x_zt * z_et - x_et * z_zt) * r;
+  met[2 * N + p] = (x_et * y_zt - x_zt * y_et) * r;
+  met[3 * N + p] = (y_zt * z_xi - y_xi * z_zt) * r;
+  met[4 * N + p] = (x_xi * z_zt - x_zt * z_xi) * r;
+  met[5 * N + p] = (x_zt * y_xi - x_xi * y_zt) * r;
+  met[6 * N + p] = (y_xi * z_et - y_et * z_xi) * r;
+  met[7 * N + p] = (x_et * z_xi - x_xi * z_et) * r;
+  met[8 * N + p] = (x_xi * y_et - x_et * y_xi) * r;
+  met[9 * N + p] = J;
+}
+
+// Mirror metrics and J into the three ghost layers above the free surface
+// (contravariant.cpp:449-463).
+__global__ void k_mirror_top(double* met, int nx, int ny, int nz, int NX, int NY, long long N) {
+  int i = H + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = H + blockIdx.y;
+  int l = blockIdx.z;  // 0..2
+  if (i >= H + nx) return;
+  const long long S = (long long)NX * NY;
+  int ks = H + nz - 1;
+  int k = ks + 1 + l;
+  long long p = (long long)i + (long long)NX * j + (long long)k * S;
+  long long q = (long long)i + (long long)NX * j + (long long)(2 * ks - k) * S;
+  for (int c = 0; c < 10; ++c) met[c * N + p] = met[c * N + q];
+}
+
+template <class R, class TM>
+__global__ void k_pack(const double* __restrict__ met, const float* __restrict__ med, TM* __restrict__ coef,
+                       R* __restrict__ jac, long long N, double stress_scale) {
+  // (Vs, Vp, rho) -> (mu, lambda, 1/rho) (CJM.cpp:53-75) packed with the metrics.
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  for (int c = 0; c < 9; ++c) coef[c * N + p] = (TM)(float)met[c * N + p];
+  double vs = med[p], vp = med[N + p], rho = med[2 * N + p];
+  double mu = rho * vs * vs, lam = rho * (vp * vp - 2.0 * vs * vs), b = 1.0 / rho;
+  coef[M_MU * N + p] = (TM)(float)(mu / stress_scale);
+  coef[M_LAM * N + p] = (TM)(float)(lam / stress_scale);
+  coef[M_B * N + p] = (TM)(float)(b * stress_scale);
+  jac[p] = (R)met[9 * N + p];
+}
+
+template <>
+__global__ void k_pack<double, double>(const double* __restrict__ met, const float* __restrict__ med,
+                                       double* __restrict__ coef, double* __restrict__ jac, long long N,
+                                       double stress_scale) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  for (int c = 0; c < 9; ++c) coef[c * N + p] = met[c * N + p];
+  double vs = med[p], vp = med[N + p], rho = med[2 * N + p];
+  coef[M_MU * N + p] = rho * vs * vs / stress_scale;
+  coef[M_LAM * N + p] = rho * (vp * vp - 2.0 * vs * vs) / stress_scale;
+  coef[M_B * N + p] = (1.0 / rho) * stress_scale;
+  jac[p] = met[9 * N + p];
+}
+
+// Traction-free condition: dV/dzeta = -DZ^-1 (DX dV/dxi + DY dV/deta) on the
+// surface (contravariant.cpp:363-447), with the 3x3 inverse in FP64.
+template <class R>
+__global__ void k_rdz(const double* __restrict__ met, const float* __restrict__ med, R* rdzdx, R* rdzdy, int nx,
+                      int ny, int nz, int NX, int NY, long long N) {
+  int i = H + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = H + blockIdx.y;
+  if (i >= H + nx) return;
+  const long long S = (long long)NX * NY;
+  int k = H + nz - 1;
+  long long p = (long long)i + (long long)NX * j + (long long)k * S;
+  double xx = met[0 * N + p], xy = met[1 * N + p], xz = met[2 * N + p];
+  double ex = met[3 * N + p], ey = met[4 * N + p], ez = met[5 * N + p];
+  double zx = met[6 * N + p], zy = met[7 * N + p], zz = met[8 * N + p];
+  double vs = med[p], vp = med[N + p], rho = med[2 * N + p];
+  double mu = rho * vs * vs, lam = rho * (vp * vp - 2.0 * vs * vs);
+  double chi = 2.0 * lam + mu, lm = lam + mu;
+  double DZ[9], DX[9], DY[9], Ri[9];
+  DZ[0] = chi * zx * zx + mu * (zy * zy + zz * zz);
+  DZ[1] = lm * zx * zy;
+  DZ[2] = lm * zx * zz;
+  DZ[3] = DZ[1];
+  DZ[4] = chi * zy * zy + mu * (zx * zx + zz * zz);
+  DZ[5] = lm * zy * zz;
+  DZ[6] = DZ[2];
+  DZ[7] = DZ[5];
+  DZ[8] = chi * zz * zz + mu * (zx * zx + zy * zy);
+  const double ax[2] = {xx, ex}, ay[2] = {xy, ey}, az[2] = {xz, ez};
+  double* Dm[2] = {DX, DY};
+  for (int t = 0; t < 2; ++t) {
+    double* D = Dm[t];
+    double a1 = ax[t], a2 = ay[t], a3 = az[t];
+    D[0] = chi * zx * a1 + mu * (zy * a2 + zz * a3);
+    D[1] = lam * zx * a2 + mu * zy * a1;
+    D[2] = lam * zx * a3 + mu * zz * a1;
+    D[3] = lam * zy * a1 + mu * zx * a2;
+    D[4] = chi * zy * a2 + mu * (zx * a1 + zz * a3);
+    D[5] = lam * zy * a3 + mu * zz * a2;
+    D[6] = lam * zz * a1 + mu * zx * a3;
+    D[7] = lam * zz * a2 + mu * zy * a3;
+    D[8] = chi * zz * a3 + mu * (zx * a1 + zy * a2);
+  }
+  double det = DZ[0] * DZ[4] * DZ[8] + DZ[1] * DZ[5] * DZ[6] + DZ[2] * DZ[7] * DZ[3] - DZ[2] * DZ[4] * DZ[6] -
+               DZ[1] * DZ[3] * DZ[8] - DZ[0] * DZ[7] * DZ[5];
+  Ri[0] = (DZ[4] * DZ[8] - DZ[5] * DZ[7]) / det;
+  Ri[1] = (-DZ[3] * DZ[8] + DZ[5] * DZ[6]) / det;
+  Ri[2] = (DZ[3] * DZ[7] - DZ[4] * DZ[6]) / det;
+  Ri[3] = (-DZ[1] * DZ[8] + DZ[2] * DZ[7]) / det;
+  Ri[4] = (DZ[0] * DZ[8] - DZ[2] * DZ[6]) / det;
+  Ri[5] = (-DZ[0] * DZ[7] + DZ[1] * DZ[6]) / det;
+  Ri[6] = (DZ[1] * DZ[5] - DZ[2] * DZ[4]) / det;
+  Ri[7] = (-DZ[0] * DZ[5] + DZ[2] * DZ[3]) / det;
+  Ri[8] = (DZ[0] * DZ[4] - DZ[1] * DZ[3]) / det;
+  long long s = (long long)i + (long long)NX * j;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      rdzdx[(3 * r + c) * S + s] = (R)(-(Ri[3 * r] * DX[c] + Ri[3 * r + 1] * DX[3 + c] + Ri[3 * r + 2] * DX[6 + c]));
+      rdzdy[(3 * r + c) * S + s] = (R)(-(Ri[3 * r] * DY[c] + Ri[3 * r + 1] * DY[3 + c] + Ri[3 * r + 2] * DY[6 + c]));
+    }
+}
+
+// Minimum / maximum point-to-plane distance over the interior (calc_CFL.cpp:56-163),
+// one partial per block.
+__global__ void k_hrange(const float* __restrict__ coord, double* part, int nx, int ny, int nz, int NX, int NY,
+                         long long N) {
+  __shared__ double smin[256], smax[256];
+  long long tot = (long long)nx * ny * nz;
+  double lo = 1e300, hi = 0.0;
+  const long long S = (long long)NX * NY;
+  const float* X = coord;
+  const float* Y = coord + N;
+  const float* Z = coord + 2 * N;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    int i = H + (int)(e % nx);
+    int j = H + (int)((e / nx) % ny);
+    int k = H + (int)(e / ((long long)nx * ny));
+    long long p = (long long)i + (long long)NX * j + (long long)k * S;
+    double P[3] = {X[p], Y[p], Z[p]};
+    for (int a = -1; a <= 1; a += 2)
+      for (int b = -1; b <= 1; b += 2)
+        for (int c = -1; c <= 1; c += 2) {
+          long long qa = p - a, qb = p - (long long)b * NX, qc = p - (long long)c * S;
+          double A[3] = {X[qa], Y[qa], Z[qa]}, B[3] = {X[qb], Y[qb], Z[qb]}, C[3] = {X[qc], Y[qc], Z[qc]};
+          double ab0 = B[0] - A[0], ab1 = B[1] - A[1], ab2 = B[2] - A[2];
+          double ac0 = C[0] - A[0], ac1 = C[1] - A[1], ac2 = C[2] - A[2];
+          double n0 = ab1 * ac2 - ab2 * ac1, n1 = ab2 * ac0 - ab0 * ac2, n2 = ab0 * ac1 - ab1 * ac0;
+          double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+          double d = fabs(((P[0] - A[0]) * n0 + (P[1] - A[1]) * n1 + (P[2] - A[2]) * n2) / nn);
+          lo = fmin(lo, d);
+          hi = fmax(hi, d);
+        }
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = smin[0];
+    part[2 * blockIdx.x + 1] = smax[0];
+  }
+}
+
+// min/max of one float array, one partial per block
+__global__ void k_frange(const float* __restrict__ a, long long n, double* part) {
+  __shared__ double smin[256], smax[256];
+  double lo = 1e300, hi = -1e300;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    double v = a[e];
+    lo = fmin(lo, v);
+    hi = fmax(hi, v);
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = smin[0];
+    part[2 * blockIdx.x + 1] = smax[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_metrics(const float* coord, double* met, int nx, int ny, int nz, int NX, int NY, long long N, double dh,
+                    bool fs, hipStream_t st) {
+  dim3 g((nx + 63) / 64, ny, nz);
+  k_metrics<<<g, 64, 0, st>>>(coord, met, nx, ny, nz, NX, NY, N, 1.0 / dh);
+  if (fs) {
+    dim3 g2((nx + 63) / 64, ny, 3);
+    k_mirror_top<<<g2, 64, 0, st>>>(met, nx, ny, nz, NX, NY, N);
+  }
+}
+
+template <class R, class TM>
+void launch_pack(const double* met, const float* med, TM* coef, R* jac, long long N, double stress_scale,
+                 hipStream_t st) {
+  k_pack<R, TM><<<(unsigned)((N + 255) / 256), 256, 0, st>>>(met, med, coef, jac, N, stress_scale);
+}
+
+template <class R>
+void launch_rdz(const double* met, const float* med, R* rdzdx, R* rdzdy, int nx, int ny, int nz, int NX, int NY,
+                long long N, hipStream_t st) {
+  dim3 g((nx + 63) / 64, ny, 1);
+  k_rdz<R><<<g, 64, 0, st>>>(met, med, rdzdx, rdzdy, nx, ny, nz, NX, NY, N);
+}
+
+void launch_hrange(const float* coord, double* part, int nblocks, int nx, int ny, int nz, int NX, int NY, long long N,
+                   hipStream_t st) {
+  k_hrange<<<nblocks, 256, 0, st>>>(coord, part, nx, ny, nz, NX, NY, N);
+}
+
+void launch_frange(const float* a, long long n, double* part, int nblocks, hipStream_t st) {
+  k_frange<<<nblocks, 256, 0, st>>>(a, n, part);
+}
+
+template void launch_pack<float, float>(const double*, const float*, float*, float*, long long, double, hipStream_t);
+template void launch_pack<float, __half>(const double*, const float*, __half*, float*, long long, double, hipStream_t);
+template void launch_pack<float, __hip_bfloat16>(const double*, const float*, __hip_bfloat16*, float*, long long,
+                                                  double, hipStream_t);
+template void launch_pack<double, double>(const double*, const float*, double*, double*, long long, double,
+                                          hipStream_t);
+template void launch_rdz<float>(const double*, const float*, float*, float*, int, int, int, int, int, long long,
+                                hipStream_t);
+template void launch_rdz<double>(const double*, const float*, double*, double*, int, int, int, int, int, long long,
+                                 hipStream_t);
+
+}  // namespace cg

@@ -1,0 +1,7 @@
+// Instantiation of the CGFDM step kernels: wavefield float, coefficients __hip_bfloat16.
+#include "cgfdm_step_impl.h"
+
+namespace cg {
+template void launch_interior<float, __hip_bfloat16>(StageArgs<float>, const __hip_bfloat16*, int, int, int, hipStream_t);
+template void launch_surface<float, __hip_bfloat16>(StageArgs<float>, const __hip_bfloat16*, SurfArgs<float>, int, int, int, hipStream_t);
+}  // namespace cg

@@ -1,0 +1,7 @@
+// Instantiation of the CGFDM step kernels: wavefield float, coefficients __half.
+#include "cgfdm_step_impl.h"
+
+namespace cg {
+template void launch_interior<float, __half>(StageArgs<float>, const __half*, int, int, int, hipStream_t);
+template void launch_surface<float, __half>(StageArgs<float>, const __half*, SurfArgs<float>, int, int, int, hipStream_t);
+}  // namespace cg

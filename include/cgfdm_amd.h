@@ -1,0 +1,119 @@
+/* C ABI of the MI355X CGFDM3D-EQR-SC time-stepping library (libcgfdm_amd.so).
+ *
+ * The reference has no library interface: its boundary for this path is the set
+ * of host launchers that propagate() calls once per step and per RK stage
+ * (reference src/propagate.cpp:186-321, declared in src/functions.h:88-216),
+ * plus the setup stage that builds their inputs (src/run.cpp:56-117). Each entry
+ * point below names the reference functions it replaces. Differences by design:
+ *   - an opaque handle instead of globals (reference run.cpp:20-24);
+ *   - int status codes instead of MPI_Abort (same numeric codes);
+ *   - one GPU, one rank: the domain is the whole grid at its real shape, so
+ *     the halo exchange (MPI_send_recv_FLOAT.cpp:388-505) disappears;
+ *   - work is enqueued on one HIP stream (cgfdm_set_stream); calls are not
+ *     thread safe on the same handle.
+ * Arrays are host memory unless stated otherwise. Grid arrays are SoA over the
+ * padded grid, x fastest: a[c][k][j][i], NX = nx + 6 etc. (HALO = 3).
+ */
+#ifndef CGFDM_AMD_H
+#define CGFDM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (reference MPI_Abort codes where one exists) */
+#define CGFDM_OK 0
+#define CGFDM_ERR_ARG 2          /* invalid argument or call order (no reference counterpart) */
+#define CGFDM_ERR_HIP 3          /* HIP runtime error (reference CHECK, macro.h:94-103, prints and continues) */
+#define CGFDM_ERR_NO_PARAMS 100  /* params.json missing (getParams.cpp:25-29) */
+#define CGFDM_ERR_PARSE 1001     /* params.json / station.json parse error (getParams.cpp:48-53) */
+#define CGFDM_ERR_CFL 110        /* DT > 1.34 Hmin / Vp_max (calc_CFL.cpp:325,341-345) */
+#define CGFDM_ERR_PML 130        /* nPML >= local n (pml_deriv.cpp:55-85) */
+#define CGFDM_ERR_SOURCE 180     /* inconsistent source description (modelChecking.cpp:23-50) */
+#define CGFDM_ERR_ALLOC 10001    /* device allocation failure (wave_deriv.cpp:49-53) */
+
+#define CGFDM_SCHEME_CGFDM 0     /* Makefile_CGFDM: MacCormack + RK4 + CFS-PML + traction image */
+#define CGFDM_SCHEME_AWENO 1     /* Makefile_SCFDM: WENO/LF alternative flux + TVD-RK3 + sponge */
+
+typedef struct cgfdm_solver cgfdm_solver;
+
+typedef struct cgfdm_desc {
+  int nx, ny, nz;     /* interior cells of the domain (grid.nx/ny/nz, init_grid.cpp:37-45 with P = 1) */
+  int npml;           /* params.nPML (getParams.cpp:137-138); 0 = no absorbing layer */
+  int free_surface;   /* 1 = FREE_SURFACE build (traction image on the top face) */
+  int scheme;         /* CGFDM_SCHEME_* */
+  int precision;      /* 4 = FP32 wavefield (reference FLOAT), 8 = FP64 wavefield */
+  int coef_storage;   /* 0 = wavefield precision, 1 = fp16, 2 = bf16 storage of metric and medium
+                         planes, with stress carried as T / (rho_ref Vp_ref) (macro.h:75-80 Cs) */
+  int nt;             /* steps the station buffers hold, NT = TMAX / DT (propagate.cpp:88) */
+  double dh, dt;      /* params.DH, params.DT */
+} cgfdm_desc;
+
+/* Setup: FP64 metrics/Jacobian (contravariant.cpp:182-285), metric mirror and
+ * traction matrices (contravariant.cpp:321-519), (Vs,Vp,rho) -> (mu,lambda,1/rho)
+ * and CJM packing (CJM.cpp:53-181), CFL check (calc_CFL.cpp:255-349), PML
+ * profiles (init_pml_para.cpp:97-285) and wave/aux allocation (wave_deriv.cpp:31-69,
+ * pml_deriv.cpp:17-124). Replaces run.cpp:75-111.
+ * coord: float[3][NZ][NY][NX] (x, y, z in metres, as constructCoord, coord.cpp:259-333)
+ * medium: float[3][NZ][NY][NX] (Vs, Vp, rho, as constructMedium, medium.cpp:251-317) */
+int cgfdm_create(const cgfdm_desc* desc, const float* coord, const float* medium, cgfdm_solver** out);
+
+/* Finite-fault moment-rate table after solveMomentRate (multiSource.cpp:450-535):
+ * rate[npts][nt][6] = (Mxx,Myy,Mzz,Mxy,Mxz,Myz) * area * slip-rate, cell[npts] = padded
+ * linear cell index. Points sharing a cell are merged (multiSource.cpp:537-592), the
+ * table is made time-major (multiSource.cpp:706-752) and scaled by mu/(J DH^3)
+ * (addMoment.cpp:27-98). Replaces init_MultiSource (multiSource.cpp:780-868) after
+ * the projection step and calculateMomentRate. */
+int cgfdm_set_source(cgfdm_solver* s, int npts, int nt, double dt, const long long* cell, const float* rate);
+
+/* Station cells as padded (i, j, k) triples (station.cpp:92-301 after parsing). */
+int cgfdm_set_stations(cgfdm_solver* s, int ns, const int* ijk);
+
+/* Advance nsteps full time steps: source injection at t = it*DT, RK stages with
+ * derivative, free surface, PML and update, then station/PGV/displacement
+ * recording (propagate.cpp:186-321). Asynchronous on the solver stream. */
+int cgfdm_step(cgfdm_solver* s, int nsteps);
+
+/* Use an existing HIP stream (hipStream_t passed as void*); NULL = default stream. */
+int cgfdm_set_stream(cgfdm_solver* s, void* stream);
+int cgfdm_synchronize(cgfdm_solver* s);
+/* number of steps taken so far */
+int cgfdm_step_index(cgfdm_solver* s);
+
+/* Station record float[ns][nt][9] (Vx,Vy,Vz,Txx,Tyy,Tzz,Txy,Txz,Tyz), the device
+ * buffer that station.cpp:409-443 copies back and writes. */
+int cgfdm_get_stations(cgfdm_solver* s, float* out);
+/* PGV map float[ny][nx][2] (PGVh, PGV) of the surface layer (PGV.cpp:62-239). */
+int cgfdm_get_pgv(cgfdm_solver* s, float* out);
+/* Surface displacement float[3][ny][nx] (FreeSurfU*, solve_displacement.cpp:28-74). */
+int cgfdm_get_surface_displacement(cgfdm_solver* s, float* out);
+/* One wavefield component of the current state, double[NZ][NY][NX] in physical
+ * units; source of the slice output (data_io.cpp:485-587). */
+int cgfdm_get_field(cgfdm_solver* s, int comp, double* out);
+/* Overwrite one wavefield component (initial conditions; no reference counterpart). */
+int cgfdm_set_field(cgfdm_solver* s, int comp, const double* in);
+
+/* Setup products for inspection: what = 0..8 metrics, 9 J, 10 mu, 11 lambda, 12 1/rho
+ * (double[NZ][NY][NX]); 20..28 rDZ_DX, 30..38 rDZ_DY (double[NY][NX]); 40 + 3*axis +
+ * {0 alpha, 1 1/beta, 2 d/beta} (double[NX|NY|NZ]). (CJM.cpp:107-147, init_pml_para.cpp) */
+int cgfdm_get_setup(cgfdm_solver* s, int what, double* out);
+/* [Hmin, Hmax, Vs_min, Vs_max, Vp_min, Vp_max, rho_min, rho_max] (calc_CFL.cpp:316-340). */
+int cgfdm_get_ranges(cgfdm_solver* s, double* out8);
+
+/* Per-kernel timing with HIP events on the solver stream. enable = 1 starts
+ * accumulation (and clears it), 0 stops. out[0..5] = {interior ms, interior
+ * launches, surface ms, surface launches, step ms, steps}; out may be NULL. */
+int cgfdm_timing(cgfdm_solver* s, int enable, double* out);
+
+/* Device bytes held by the solver. */
+long long cgfdm_device_bytes(cgfdm_solver* s);
+
+/* Last error message of this thread. */
+const char* cgfdm_last_error(void);
+
+int cgfdm_destroy(cgfdm_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,167 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build(force=False):
+    so = os.path.join(_HERE, "liboracle.so")
+    src = os.path.join(_HERE, "cgfdm_oracle.cpp")
+    if force or not os.path.exists(so) or os.path.getmtime(so) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "all"])
+    return so
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(so):
+            build()
+        L = ctypes.CDLL(so)
+        vp, i, d, ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [i, i, i, i, i, i, d, d, i, vp, vp]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_set_source.argtypes = [vp, i, i, d, vp, vp]
+        L.orc_set_stations.argtypes = [vp, i, vp]
+        L.orc_set_threads.argtypes = [i]
+        L.orc_step.argtypes = [vp, i]
+        L.orc_get_field.argtypes = [vp, i, i, vp]
+        L.orc_set_field.argtypes = [vp, i, vp]
+        L.orc_get_setup.argtypes = [vp, i, vp]
+        L.orc_get_ranges.argtypes = [vp, vp]
+        L.orc_get_stations.argtypes = [vp, vp]
+        L.orc_get_pgv.argtypes = [vp, vp]
+        L.orc_get_surface_dis.argtypes = [vp, vp]
+        L.orc_face_count.restype = ll
+        L.orc_face_count.argtypes = [vp, i]
+        L.orc_get_face.argtypes = [vp, i, i, vp]
+        L.orc_rhs.argtypes = [vp, i, i, i, i]
+        L.orc_rk_stage.argtypes = [i, i, ll, vp, vp, vp, vp]
+        L.orc_step_index.restype = i
+        L.orc_step_index.argtypes = [vp]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    """CPU restatement of the CGFDM step on a `Model` (prec 4 = float, 8 = double)."""
+
+    def __init__(self, model, prec=8, threads=None):
+        self.L = lib()
+        self.m = model
+        self.prec = prec
+        if threads:
+            self.L.orc_set_threads(int(threads))
+        self._coord = np.ascontiguousarray(model.coord, np.float32)
+        self._med = np.ascontiguousarray(model.medium, np.float32)
+        self.h = self.L.orc_create(prec, model.nx, model.ny, model.nz, model.npml,
+                                   1 if model.free_surface else 0, float(model.dh), float(model.dt),
+                                   int(model.nt), _ptr(self._coord), _ptr(self._med))
+        NZ, NY, NX = model.shape_padded
+        self.shape = (NZ, NY, NX)
+        self.N = NZ * NY * NX
+        if model.src_cells is not None and len(model.src_cells):
+            cells = np.ascontiguousarray(model.src_cells, np.int64)
+            rate = np.ascontiguousarray(model.src_rate, np.float32)
+            self.L.orc_set_source(self.h, len(cells), rate.shape[1], float(model.src_dt), _ptr(cells), _ptr(rate))
+        self.ns = 0
+        if model.stations is not None and len(model.stations):
+            st = np.ascontiguousarray(model.stations, np.int32)
+            self.ns = len(st)
+            self.L.orc_set_stations(self.h, self.ns, _ptr(st))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.L.orc_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        self.L.orc_step(self.h, int(n))
+
+    @property
+    def it(self):
+        return self.L.orc_step_index(self.h)
+
+    def field(self, comp, which=0):
+        out = np.empty(self.N, np.float64)
+        self.L.orc_get_field(self.h, which, comp, _ptr(out))
+        return out.reshape(self.shape)
+
+    def wavefield(self, which=0):
+        return np.stack([self.field(c, which) for c in range(9)])
+
+    def set_field(self, comp, arr):
+        a = np.ascontiguousarray(arr, np.float64).ravel()
+        self.L.orc_set_field(self.h, comp, _ptr(a))
+
+    def setup(self, what):
+        NZ, NY, NX = self.shape
+        if what < 13:
+            n, shp = self.N, self.shape
+        elif what < 40:
+            n, shp = NY * NX, (NY, NX)
+        else:
+            ax = (what - 40) // 3
+            n = (NX, NY, NZ)[ax]
+            shp = (n,)
+        out = np.empty(n, np.float64)
+        self.L.orc_get_setup(self.h, what, _ptr(out))
+        return out.reshape(shp)
+
+    def ranges(self):
+        out = np.empty(8, np.float64)
+        self.L.orc_get_ranges(self.h, _ptr(out))
+        return dict(zip(["hmin", "hmax", "vsmin", "vsmax", "vpmin", "vpmax", "rhomin", "rhomax"], out))
+
+    def stations(self):
+        out = np.zeros((self.ns, self.m.nt, 9), np.float32)
+        if self.ns:
+            self.L.orc_get_stations(self.h, _ptr(out))
+        return out
+
+    def pgv(self):
+        out = np.zeros((self.m.ny, self.m.nx, 2), np.float32)
+        self.L.orc_get_pgv(self.h, _ptr(out))
+        return out
+
+    def surface_dis(self):
+        NZ, NY, NX = self.shape
+        out = np.zeros((3, NY, NX), np.float64)
+        self.L.orc_get_surface_dis(self.h, _ptr(out))
+        return out
+
+    def rhs(self, fb, with_boundaries=True):
+        self.L.orc_rhs(self.h, int(fb[0]), int(fb[1]), int(fb[2]), 1 if with_boundaries else 0)
+        return self.wavefield(which=3)
+
+    def face(self, f, comp):
+        n = self.L.orc_face_count(self.h, f)
+        out = np.empty(n, np.float64)
+        self.L.orc_get_face(self.h, f, comp, _ptr(out))
+        return out
+
+
+def rk_stage(prec, irk, h, w, t, m):
+    """Apply one low-storage RK4 stage combination in place on flat arrays."""
+    L = lib()
+    L.orc_rk_stage(prec, irk, h.size, _ptr(h), _ptr(w), _ptr(t), _ptr(m))
