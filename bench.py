@@ -92,7 +92,8 @@ def main():
     from cgfdm_amd import solver as S
 
     t0 = time.perf_counter()
-    m = M.config2(nt=args.warmup + args.steps + 1, nx=args.nx, ny=args.ny, nz=args.nz)
+    hmin_gpu = lambda c, md: S.grid_ranges(args.nx, args.ny, args.nz, c, md)["hmin"]
+    m = M.config2(nt=args.warmup + args.steps + 1, nx=args.nx, ny=args.ny, nz=args.nz, hmin_fn=hmin_gpu)
     log(f"model {m.nx}x{m.ny}x{m.nz} built in {time.perf_counter() - t0:.1f}s, dt={m.dt:.4e}")
 
     dev = torch.device("cuda:0")

@@ -14,7 +14,7 @@ LIB = os.path.join(PKG, "libcgfdm_amd.so")
 OBJ = os.path.join(ROOT, "_build")
 
 SOURCES = ["setup.hip", "step_fp32.hip", "step_fp64.hip", "step_fp16.hip", "step_bf16.hip", "capi.cpp"]
-HEADERS = ["common.h", "kernels.h", "setup.h", "cgfdm_step_impl.h"]
+HEADERS = ["common.h", "kernels.h", "setup.h", "cgfdm_step_impl.h", "launch_impl.h"]  # cgfdm_zmarch_disabled.h is not built
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

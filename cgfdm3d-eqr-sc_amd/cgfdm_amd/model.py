@@ -131,9 +131,10 @@ def layered_medium(coord, terrain, tops_m, vs, vp, rho, tolerance=1.0):
 
 def gaussian_random_field(shape, dh, corr_len, sigma, seed, clip=3.0):
     """Zero-mean Gaussian field with Gaussian autocorrelation (FFT-filtered noise)."""
+    import scipy.fft as sfft
     rng = np.random.default_rng(seed)
-    noise = rng.standard_normal(shape).astype(np.float32)
-    spec = np.fft.rfftn(noise)
+    noise = rng.standard_normal(shape, dtype=np.float32)
+    spec = sfft.rfftn(noise, workers=-1)
     ks = [np.fft.fftfreq(n, d=dh) for n in shape[:-1]] + [np.fft.rfftfreq(shape[-1], d=dh)]
     filt = np.ones(spec.shape, np.float32)
     for ax, k in enumerate(ks):
@@ -141,7 +142,8 @@ def gaussian_random_field(shape, dh, corr_len, sigma, seed, clip=3.0):
         sh[ax] = len(k)
         # autocorrelation exp(-r^2/a^2) <=> spectrum exp(-(pi a k)^2); amplitude = sqrt
         filt = filt * np.exp(-0.5 * (math.pi * corr_len * k) ** 2).astype(np.float32).reshape(sh)
-    f = np.fft.irfftn(spec * filt, s=shape, axes=tuple(range(len(shape)))).astype(np.float32)
+    spec *= filt
+    f = sfft.irfftn(spec, s=shape, workers=-1).astype(np.float32)
     f *= sigma / max(float(f.std()), 1e-30)
     return np.clip(f, -clip * sigma, clip * sigma).astype(np.float32)
 
@@ -251,7 +253,7 @@ def config1(nt=1000, nx=128, ny=128, nz=64, dh=100.0, npml=12):
 
 
 def config2(nt=3000, nx=512, ny=512, nz=256, dh=100.0, npml=20, seed=1002, nhills=8,
-            perturb=True):
+            perturb=True, hmin_fn=None):
     """CGFDM curvilinear: 8 seeded Gaussian hills, 4-layer medium with a 5%
     Gaussian random perturbation (1 km correlation), 20-layer PML, point
     moment source at 5 km depth, 400 surface stations, PGV map."""
@@ -265,7 +267,8 @@ def config2(nt=3000, nx=512, ny=512, nz=256, dh=100.0, npml=20, seed=1002, nhill
         delta = gaussian_random_field(med.shape[1:], dh, 1000.0, 0.05, seed + 7)
         med[0] *= 1.0 + delta
         med[1] *= 1.0 + delta
-    hmin = hmin_point_plane(coord)
+        del delta
+    hmin = (hmin_fn or (lambda c, md: hmin_point_plane(c)))(coord, med)
     dt = 0.5 * hmin / float(med[1].max())
     m = Model(nx, ny, nz, dh, dt, nt, npml, coord, med, True, name="config2", terrain=terrain)
     depth_cells = int(round(5000.0 / dh))

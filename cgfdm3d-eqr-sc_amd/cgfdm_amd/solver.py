@@ -41,7 +41,7 @@ EXPORTS = ["cgfdm_create", "cgfdm_destroy", "cgfdm_set_source", "cgfdm_set_stati
            "cgfdm_set_stream", "cgfdm_synchronize", "cgfdm_step_index", "cgfdm_get_stations",
            "cgfdm_get_pgv", "cgfdm_get_surface_displacement", "cgfdm_get_field", "cgfdm_set_field",
            "cgfdm_get_setup", "cgfdm_get_ranges", "cgfdm_timing", "cgfdm_device_bytes",
-           "cgfdm_last_error"]
+           "cgfdm_last_error", "cgfdm_grid_ranges"]
 
 
 def lib():
@@ -72,6 +72,7 @@ def lib():
     L.cgfdm_device_bytes.restype = ll
     L.cgfdm_device_bytes.argtypes = [vp]
     L.cgfdm_last_error.restype = ctypes.c_char_p
+    L.cgfdm_grid_ranges.argtypes = [i, i, i, vp, vp, vp]
     _lib = L
     return L
 
@@ -83,6 +84,16 @@ def _ptr(a):
 def _check(rc):
     if rc != 0:
         raise CgfdmError(rc, lib().cgfdm_last_error().decode())
+
+
+def grid_ranges(nx, ny, nz, coord, medium):
+    """FP64 GPU evaluation of the CFL inputs of a grid (Hmin, Hmax and medium ranges)."""
+    L = lib()
+    c = np.ascontiguousarray(coord, np.float32)
+    m = np.ascontiguousarray(medium, np.float32)
+    out = np.empty(8, np.float64)
+    _check(L.cgfdm_grid_ranges(int(nx), int(ny), int(nz), _ptr(c), _ptr(m), _ptr(out)))
+    return dict(zip(["hmin", "hmax", "vsmin", "vsmax", "vpmin", "vpmax", "rhomin", "rhomax"], out))
 
 
 class Solver:

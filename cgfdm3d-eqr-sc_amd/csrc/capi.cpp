@@ -1,6 +1,6 @@
 // C ABI of the MI355X CGFDM library: solver lifetime, setup, time stepping
 // and observers. Host orchestration only; all grid work runs in the HIP
-// kernels of setup.hip and cgfdm_step.hip.
+// kernels of setup.hip and step_*.hip.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -86,7 +86,6 @@ struct Solver : SolverBase {
   double src_dt = 0;
   long long* src_cell = nullptr;
   R* src_rate = nullptr;
-  int kchunk = 64;
   // timing
   bool timing_on = false;
   struct EvPair {
@@ -135,7 +134,6 @@ struct Solver : SolverBase {
     NZ = d.nz + 2 * cg::H;
     S = (long long)NX * NY;
     N = S * NZ;
-    if (const char* e = std::getenv("CGFDM_KCHUNK")) kchunk = std::max(4, std::atoi(e));
     int rc;
     // --- temporaries: coordinates, medium, FP64 metrics
     float *dcoord = nullptr, *dmed = nullptr;
@@ -375,7 +373,6 @@ struct Solver : SolverBase {
         a.S = S;
         a.kbeg = cg::H;
         a.kend = kend;
-        a.kc = kchunk;
         a.stage = irk;
         a.DT = R(d.dt);
         a.rDH = R(1.0 / d.dh);
@@ -557,6 +554,46 @@ int cgfdm_create(const cgfdm_desc* desc, const float* coord, const float* medium
   auto* h = new cgfdm_solver();
   h->impl = std::move(s);
   *out = h;
+  return CGFDM_OK;
+}
+
+int cgfdm_grid_ranges(int nx, int ny, int nz, const float* coord, const float* medium, double* out8) {
+  if (nx < 1 || ny < 1 || nz < 1 || !coord || !medium || !out8) return fail(CGFDM_ERR_ARG, "bad arguments");
+  const int NX = nx + 2 * cg::H, NY = ny + 2 * cg::H, NZ = nz + 2 * cg::H;
+  const long long N = (long long)NX * NY * NZ;
+  const int nb = 1024;
+  float *dc = nullptr, *dm = nullptr;
+  double* part = nullptr;
+  auto cleanup = [&]() {
+    if (dc) hipFree(dc);
+    if (dm) hipFree(dm);
+    if (part) hipFree(part);
+  };
+  if (hipMalloc(&dc, sizeof(float) * 3 * N) != hipSuccess || hipMalloc(&dm, sizeof(float) * 3 * N) != hipSuccess ||
+      hipMalloc(&part, sizeof(double) * 8 * nb) != hipSuccess) {
+    cleanup();
+    return fail(CGFDM_ERR_ALLOC, "hipMalloc failed");
+  }
+  std::vector<double> h(8 * nb);
+  hipError_t e = hipMemcpy(dc, coord, sizeof(float) * 3 * N, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dm, medium, sizeof(float) * 3 * N, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    cg::launch_hrange(dc, part, nb, nx, ny, nz, NX, NY, N, nullptr);
+    for (int c = 0; c < 3; ++c) cg::launch_frange(dm + c * N, N, part + 2 * nb * (c + 1), nb, nullptr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost);
+  cleanup();
+  if (e != hipSuccess) return fail(CGFDM_ERR_HIP, hipGetErrorString(e));
+  for (int c = 0; c < 4; ++c) {
+    double lo = 1e300, hi = -1e300;
+    for (int b = 0; b < nb; ++b) {
+      lo = std::min(lo, h[2 * nb * c + 2 * b]);
+      hi = std::max(hi, h[2 * nb * c + 2 * b + 1]);
+    }
+    out8[2 * c] = lo;
+    out8[2 * c + 1] = hi;
+  }
   return CGFDM_OK;
 }
 

@@ -2,20 +2,18 @@
 //
 // One RK4 stage of the reference (propagate.cpp:212-251: wave_deriv ->
 // free_surface_deriv -> pml_deriv -> pml_free_surface -> wave_rk -> pml_rk)
-// is done here by two fused kernels that each read the stencil input once and
-// write the stage output once:
+// is done by two fused kernels, one thread per cell, that read the stencil
+// input and write the stage output once:
 //
-//   cg_interior  every cell below the three surface layers. A block owns a
-//                TX x TY column tile and marches up a chunk of k. The 5-point
-//                z stencil lives in a register queue (one new plane per step),
-//                the x/y stencils come from a double-buffered LDS copy of the
-//                current plane. Derivatives, the 27-term right-hand side, the
-//                CFS-PML auxiliary ODE of every face the cell belongs to and
-//                the RK4 update of W and of the auxiliaries are fused.
+//   cg_cell      every cell below the three surface layers: 27 MacCormack
+//                derivatives, the right-hand side, the CFS-PML auxiliary ODE of
+//                every face the cell belongs to and the RK4 update of W and of
+//                the auxiliaries. Rows of 64 cells along x make every load and
+//                store one coalesced 256 B (FP32) transaction per wavefront.
 //   cg_surface   the top three layers (traction image, reference
 //                freeSurface.cpp:243-494 and pml_freeSurface.cpp:17-242),
-//                same fusion, one thread per cell; on the last stage it also
-//                updates the PGV map and the surface displacement.
+//                same fusion; on the last stage it also updates the PGV map and
+//                the surface displacement.
 //
 // RK4 buffers (classical RK4 in the reference's (W, t, m) form, wave_rk.cpp):
 //   stage 0: in = A (also the RK base m), out = B, t = A + h/6
@@ -70,14 +68,15 @@ __device__ __forceinline__ R mac(const R* f, long long s) {
 // pml_deriv.cpp:235-273 and pml_rk.cpp:19-73): hA = (d Q - (alpha+d) A) DT,
 // h -= beta A DT, then the RK4 stage on the auxiliary state.
 template <class R>
-__device__ __forceinline__ void pml_face(const FaceBuf<R>& F, long long pos, const R* Q, R d, R ad, R be, R DT,
-                                         int stage, R* h, const R* extra /* nullable: added to hA[3..8] */) {
+__device__ __forceinline__ void pml_face(const FaceBuf<R>& F, long long pos, const R (&Q)[9], R d, R ad, R be, R DT,
+                                         int stage, R (&h)[9], bool has_extra, const R (&extra)[6]) {
+  // extra: surface stress term added to hA[3..8] on the top layer
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
     long long q = (long long)v * F.cnt + pos;
     R cur = stage == 0 ? F.a[q] : F.b[q];
     R hA = (d * Q[v] - ad * cur) * DT;
-    if (extra != nullptr && v >= 3) hA += extra[v - 3] * DT;
+    if (has_extra && v >= 3) hA += extra[v - 3] * DT;
     h[v] -= be * cur * DT;
     if (stage == 0) {
       F.t[q] = cur + R(RK_B1) * hA;
@@ -95,198 +94,97 @@ __device__ __forceinline__ void pml_face(const FaceBuf<R>& F, long long pos, con
 }
 
 // ---------------------------------------------------------------------------
-// interior kernel
+// interior kernel: one thread per cell below the surface layers
 // ---------------------------------------------------------------------------
-template <class R, class TM, int FB1, int FB2, int FB3, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void cg_interior(StageArgs<R> a, const TM* __restrict__ coef) {
-  constexpr int LX = TX + 6, LY = TY + 6, LP = LX * LY;  // LDS plane with 3 ghost cells per side
-  constexpr int NT = TX * TY;
-  constexpr int HXL = FB1 > 0 ? 1 : 3, HXR = 4 - HXL;     // x halo actually needed (left/right)
-  constexpr int HYL = FB2 > 0 ? 1 : 3, HYR = 4 - HYL;
-  constexpr int NH = 4 * TY + 4 * TX;                     // halo entries per field
-  constexpr int NHR = (9 * NH + NT - 1) / NT;             // halo entries per thread
-  constexpr int LO = FB3 > 0 ? -1 : -3;                   // z queue holds planes k+LO .. k+LO+4
-  constexpr int CZ = -LO;                                 // queue slot of plane k
-  __shared__ R lds[2][9][LP];
-
-  // XCD-aware tile order: hardware hands consecutive block ids to the 8 XCDs in
-  // turn; give each XCD a contiguous run of tiles so halo re-reads hit its L2.
-  const int nb = gridDim.x;
-  int bid = blockIdx.x;
-  int logical = (bid & 7) * (nb >> 3) + (bid >> 3);
-  if (logical >= a.ntiles) return;  // uniform per block, before any barrier
-  const int tile_x = logical % a.ntx;
-  const int rest = logical / a.ntx;
-  const int tile_y = rest % a.nty;
-  const int chunk = rest / a.nty;
-
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-  const int i = H + tile_x * TX + tx;
-  const int j = H + tile_y * TY + ty;
-  const bool in_pad = (i < a.NX) && (j < a.NY);
-  const bool active = (i < H + a.nx) && (j < H + a.ny);
-  const int kbeg = a.kbeg + chunk * a.kc;
-  const int kend = min(kbeg + a.kc, a.kend);
+template <class R, class TM, int FB1, int FB2, int FB3>
+__global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restrict__ coef) {
+  const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k = a.kbeg + blockIdx.z;
+  if (i >= H + a.nx || j >= H + a.ny || k >= a.kend) return;
   const long long N = a.N, S = a.S;
-  const long long col = (long long)i + (long long)a.NX * j;
+  const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
+  const R* W = a.win;
+  auto cf = [&](int c) -> R { return R(Store<TM>::ld(coef + (long long)c * N + p)); };
+  const R mu = cf(M_MU), lam = cf(M_LAM), bu = cf(M_B);
+  const PmlArgs<R>& P = a.pml;
+  R h[9], D[9], Q[9];
 
-  // halo bookkeeping: entry e -> (field, LDS slot, column offset); -1 = none
-  int h_lds[NHR];
-  long long h_col[NHR];
-  int h_fld[NHR];
+  // x: derivatives along xi (wave_deriv.cpp:214-222), scaled by 1/beta_x
 #pragma unroll
-  for (int r = 0; r < NHR; ++r) {
-    int e = threadIdx.x + r * NT;
-    h_lds[r] = -1;
-    h_col[r] = 0;
-    h_fld[r] = 0;
-    if (e < 9 * NH) {
-      int v = e / NH, q = e % NH;
-      int gx, gy;  // tile-local coordinates of the halo cell (may be negative)
-      if (q < 4 * TY) {  // x halo: 4 columns x TY rows
-        int c = q / TY, rr = q % TY;
-        gx = c < HXL ? -HXL + c : TX + (c - HXL);
-        gy = rr;
-      } else {           // y halo: 4 rows x TX columns
-        int q2 = q - 4 * TY;
-        int rr = q2 / TX, c = q2 % TX;
-        gy = rr < HYL ? -HYL + rr : TY + (rr - HYL);
-        gx = c;
-      }
-      int gi = H + tile_x * TX + gx, gj = H + tile_y * TY + gy;
-      h_fld[r] = v;
-      h_lds[r] = (gy + 3) * LX + (gx + 3);
-      if (gi >= 0 && gi < a.NX && gj >= 0 && gj < a.NY) h_col[r] = (long long)gi + (long long)a.NX * gj;
-      else h_col[r] = -1;
-    }
-  }
-
-  R zq[9][5];
-  R nxt[9];
-  R hreg[NHR];
-  // prologue: planes kbeg+LO .. kbeg+LO+3 into slots 1..4, plane kbeg+LO+4 prefetched
+  for (int v = 0; v < 9; ++v) D[v] = mac<FB1>(W + (long long)v * N + p, 1);
+  assemble(D, cf(M_XIX), cf(M_XIY), cf(M_XIZ), mu, lam, bu, Q);
+  const R bx = P.beta[0][i];
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
-#pragma unroll
-    for (int m = 1; m < 5; ++m) zq[v][m] = in_pad ? a.win[(long long)v * N + col + (long long)(kbeg + LO + m - 1) * S] : R(0);
-    nxt[v] = in_pad ? a.win[(long long)v * N + col + (long long)(kbeg + LO + 4) * S] : R(0);
+    Q[v] *= a.rDH;
+    h[v] = bx * Q[v] * a.DT;
   }
-#pragma unroll
-  for (int r = 0; r < NHR; ++r)
-    hreg[r] = (h_lds[r] >= 0 && h_col[r] >= 0) ? a.win[(long long)h_fld[r] * N + h_col[r] + (long long)kbeg * S] : R(0);
-
-  const PmlArgs<R>& P = a.pml;
-  const bool in_x1 = P.npml > 0 && i < H + P.npml;
-  const bool in_x2 = P.npml > 0 && i >= H + a.nx - P.npml;
-  const bool in_y1 = P.npml > 0 && j < H + P.npml;
-  const bool in_y2 = P.npml > 0 && j >= H + a.ny - P.npml;
-  const R bx = active ? P.beta[0][i] : R(1);
-  const R by = active ? P.beta[1][j] : R(1);
-
-  int buf = 0;
-  for (int k = kbeg; k < kend; ++k) {
-#pragma unroll
-    for (int v = 0; v < 9; ++v) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m) zq[v][m] = zq[v][m + 1];
-      zq[v][4] = nxt[v];
+  const R none[6] = {0, 0, 0, 0, 0, 0};
+  if (P.npml > 0) {
+    const bool x1 = i < H + P.npml, x2 = i >= H + a.nx - P.npml;
+    if (x1 || x2) {
+      const R d = P.d[0][i], ad = d + P.alpha[0][i];
+      const long long row = (long long)(j - H) + (long long)a.ny * (k - H);
+      if (x1) pml_face<R>(P.f[0], (long long)(i - H) + (long long)P.npml * row, Q, d, ad, bx, a.DT, a.stage, h, false, none);
+      if (x2) pml_face<R>(P.f[3], (long long)(i - H - (a.nx - P.npml)) + (long long)P.npml * row, Q, d, ad, bx, a.DT, a.stage, h, false, none);
     }
-    R* L = &lds[buf][0][0];
+  }
+  // y: derivatives along eta (wave_deriv.cpp:224-232)
 #pragma unroll
-    for (int v = 0; v < 9; ++v) L[v * LP + (ty + 3) * LX + tx + 3] = zq[v][CZ];
+  for (int v = 0; v < 9; ++v) D[v] = mac<FB2>(W + (long long)v * N + p, a.NX);
+  assemble(D, cf(M_ETX), cf(M_ETY), cf(M_ETZ), mu, lam, bu, Q);
+  const R by = P.beta[1][j];
 #pragma unroll
-    for (int r = 0; r < NHR; ++r)
-      if (h_lds[r] >= 0) L[h_fld[r] * LP + h_lds[r]] = hreg[r];
-    if (k + 1 < kend) {
-#pragma unroll
-      for (int v = 0; v < 9; ++v) nxt[v] = in_pad ? a.win[(long long)v * N + col + (long long)(k + 1 + LO + 4) * S] : R(0);
-#pragma unroll
-      for (int r = 0; r < NHR; ++r)
-        hreg[r] = (h_lds[r] >= 0 && h_col[r] >= 0) ? a.win[(long long)h_fld[r] * N + h_col[r] + (long long)(k + 1) * S] : R(0);
+  for (int v = 0; v < 9; ++v) {
+    Q[v] *= a.rDH;
+    h[v] += by * Q[v] * a.DT;
+  }
+  if (P.npml > 0) {
+    const bool y1 = j < H + P.npml, y2 = j >= H + a.ny - P.npml;
+    if (y1 || y2) {
+      const R d = P.d[1][j], ad = d + P.alpha[1][j];
+      const long long i0 = i - H, k0 = k - H;
+      if (y1) pml_face<R>(P.f[1], i0 + (long long)a.nx * ((j - H) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, false, none);
+      if (y2) pml_face<R>(P.f[4], i0 + (long long)a.nx * ((j - H - (a.ny - P.npml)) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, false, none);
     }
-    __syncthreads();
-
-    if (active) {
-      const long long p = col + (long long)k * S;
-      R h[9];
-      const R mu = R(Store<TM>::ld(coef + (long long)M_MU * N + p));
-      const R lam = R(Store<TM>::ld(coef + (long long)M_LAM * N + p));
-      const R bu = R(Store<TM>::ld(coef + (long long)M_B * N + p));
-      const R bz = P.beta[2][k];
-      const int c0 = (ty + 3) * LX + tx + 3;
-      R D[9], Q[9];
-      // x direction
+  }
+  // z: derivatives along zeta (wave_deriv.cpp:234-242)
 #pragma unroll
-      for (int v = 0; v < 9; ++v) D[v] = mac<FB1>(L + v * LP + c0, 1);
-      assemble(D, R(Store<TM>::ld(coef + (long long)M_XIX * N + p)), R(Store<TM>::ld(coef + (long long)M_XIY * N + p)),
-               R(Store<TM>::ld(coef + (long long)M_XIZ * N + p)), mu, lam, bu, Q);
+  for (int v = 0; v < 9; ++v) D[v] = mac<FB3>(W + (long long)v * N + p, S);
+  assemble(D, cf(M_ZTX), cf(M_ZTY), cf(M_ZTZ), mu, lam, bu, Q);
+  const R bz = P.beta[2][k];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        Q[v] *= a.rDH;
-        h[v] = bx * Q[v] * a.DT;
-      }
-      if (in_x1 || in_x2) {
-        R d = P.d[0][i], ad = d + P.alpha[0][i];
-        long long rowq = (long long)(j - H) + (long long)a.ny * (k - H);
-        if (in_x1) pml_face<R>(P.f[0], (long long)(i - H) + (long long)P.npml * rowq, Q, d, ad, bx, a.DT, a.stage, h, nullptr);
-        if (in_x2) pml_face<R>(P.f[3], (long long)(i - H - (a.nx - P.npml)) + (long long)P.npml * rowq, Q, d, ad, bx, a.DT, a.stage, h, nullptr);
-      }
-      // y direction
-#pragma unroll
-      for (int v = 0; v < 9; ++v) D[v] = mac<FB2>(L + v * LP + c0, LX);
-      assemble(D, R(Store<TM>::ld(coef + (long long)M_ETX * N + p)), R(Store<TM>::ld(coef + (long long)M_ETY * N + p)),
-               R(Store<TM>::ld(coef + (long long)M_ETZ * N + p)), mu, lam, bu, Q);
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        Q[v] *= a.rDH;
-        h[v] += by * Q[v] * a.DT;
-      }
-      if (in_y1 || in_y2) {
-        R d = P.d[1][j], ad = d + P.alpha[1][j];
-        long long i0 = i - H, k0 = k - H;
-        if (in_y1) pml_face<R>(P.f[1], i0 + (long long)a.nx * ((j - H) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, nullptr);
-        if (in_y2) pml_face<R>(P.f[4], i0 + (long long)a.nx * ((j - H - (a.ny - P.npml)) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, nullptr);
-      }
-      // z direction (register queue)
-#pragma unroll
-      for (int v = 0; v < 9; ++v)
-        D[v] = R(FB3) * (R(AF_1) * zq[v][CZ - FB3] + R(AF0) * zq[v][CZ] + R(AF1) * zq[v][CZ + FB3] +
-                         R(AF2) * zq[v][CZ + 2 * FB3] + R(AF3) * zq[v][CZ + 3 * FB3]);
-      assemble(D, R(Store<TM>::ld(coef + (long long)M_ZTX * N + p)), R(Store<TM>::ld(coef + (long long)M_ZTY * N + p)),
-               R(Store<TM>::ld(coef + (long long)M_ZTZ * N + p)), mu, lam, bu, Q);
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        Q[v] *= a.rDH;
-        h[v] += bz * Q[v] * a.DT;
-      }
-      const bool in_z1 = P.npml > 0 && k < H + P.npml;
-      const bool in_z2 = P.has_z2 && k >= H + a.nz - P.npml;
-      if (in_z1 || in_z2) {
-        R d = P.d[2][k], ad = d + P.alpha[2][k];
-        long long i0 = i - H, j0 = j - H;
-        if (in_z1) pml_face<R>(P.f[2], i0 + (long long)a.nx * (j0 + (long long)a.ny * (k - H)), Q, d, ad, bz, a.DT, a.stage, h, nullptr);
-        if (in_z2) pml_face<R>(P.f[5], i0 + (long long)a.nx * (j0 + (long long)a.ny * (k - H - (a.nz - P.npml))), Q, d, ad, bz, a.DT, a.stage, h, nullptr);
-      }
-      // RK4 stage update of W
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        long long q = (long long)v * N + p;
-        if (a.stage == 0) {
-          R m = zq[v][CZ];
-          a.wt[q] = m + R(RK_B1) * h[v];
-          a.wout[q] = m + R(RK_A2) * h[v];
-        } else if (a.stage == 1) {
-          a.wt[q] = a.wt[q] + R(RK_B2) * h[v];
-          a.wout[q] = a.wm[q] + R(RK_A3) * h[v];
-        } else if (a.stage == 2) {
-          a.wt[q] = a.wt[q] + R(RK_B3) * h[v];
-          a.wout[q] = a.wm[q] + h[v];
-        } else {
-          a.wout[q] = a.wt[q] + R(RK_B4) * h[v];
-        }
-      }
+  for (int v = 0; v < 9; ++v) {
+    Q[v] *= a.rDH;
+    h[v] += bz * Q[v] * a.DT;
+  }
+  if (P.npml > 0) {
+    const bool z1 = k < H + P.npml, z2 = P.has_z2 && k >= H + a.nz - P.npml;
+    if (z1 || z2) {
+      const R d = P.d[2][k], ad = d + P.alpha[2][k];
+      const long long i0 = i - H, j0 = j - H;
+      if (z1) pml_face<R>(P.f[2], i0 + (long long)a.nx * (j0 + (long long)a.ny * (k - H)), Q, d, ad, bz, a.DT, a.stage, h, false, none);
+      if (z2) pml_face<R>(P.f[5], i0 + (long long)a.nx * (j0 + (long long)a.ny * (k - H - (a.nz - P.npml))), Q, d, ad, bz, a.DT, a.stage, h, false, none);
     }
-    buf ^= 1;
+  }
+  // RK4 stage update of W (wave_rk.cpp:18-104)
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    const long long q = (long long)v * N + p;
+    if (a.stage == 0) {
+      const R m = W[q];
+      a.wt[q] = m + R(RK_B1) * h[v];
+      a.wout[q] = m + R(RK_A2) * h[v];
+    } else if (a.stage == 1) {
+      a.wt[q] = a.wt[q] + R(RK_B2) * h[v];
+      a.wout[q] = a.wm[q] + R(RK_A3) * h[v];
+    } else if (a.stage == 2) {
+      a.wt[q] = a.wt[q] + R(RK_B3) * h[v];
+      a.wout[q] = a.wm[q] + h[v];
+    } else {
+      a.wout[q] = a.wt[q] + R(RK_B4) * h[v];
+    }
   }
 }
 
@@ -398,37 +296,35 @@ __global__ __launch_bounds__(256) void cg_surface(StageArgs<R> a, const TM* __re
     R Q[9];
     assemble(Dx, cf(M_XIX, p), cf(M_XIY, p), cf(M_XIZ, p), mu, lam, bu, Q);
     R d = P.d[0][i], ad = d + P.alpha[0][i];
-    R ex[6];
-    const R* pex = nullptr;
-    if (k == ktop) {
+    R ex[6] = {0, 0, 0, 0, 0, 0};
+    const bool top = (k == ktop);
+    if (top) {
       R V[3] = {Dx[0] * d, Dx[1] * d, Dx[2] * d}, Vz[3];
 #pragma unroll
       for (int r = 0; r < 3; ++r)
         Vz[r] = s.rdzdx[(3 * r) * SS + sp] * V[0] + s.rdzdx[(3 * r + 1) * SS + sp] * V[1] + s.rdzdx[(3 * r + 2) * SS + sp] * V[2];
       stress_rows(Vz, cf(M_ZTX, p), cf(M_ZTY, p), cf(M_ZTZ, p), mu, lam, ex);
-      pex = ex;
     }
     long long rowq = (long long)(j - H) + (long long)a.ny * (k - H);
-    if (in_x1) pml_face<R>(P.f[0], (long long)(i - H) + (long long)P.npml * rowq, Q, d, ad, bx, a.DT, a.stage, h, pex);
-    if (in_x2) pml_face<R>(P.f[3], (long long)(i - H - (a.nx - P.npml)) + (long long)P.npml * rowq, Q, d, ad, bx, a.DT, a.stage, h, pex);
+    if (in_x1) pml_face<R>(P.f[0], (long long)(i - H) + (long long)P.npml * rowq, Q, d, ad, bx, a.DT, a.stage, h, top, ex);
+    if (in_x2) pml_face<R>(P.f[3], (long long)(i - H - (a.nx - P.npml)) + (long long)P.npml * rowq, Q, d, ad, bx, a.DT, a.stage, h, top, ex);
   }
   if (in_y1 || in_y2) {
     R Q[9];
     assemble(Dy, cf(M_ETX, p), cf(M_ETY, p), cf(M_ETZ, p), mu, lam, bu, Q);
     R d = P.d[1][j], ad = d + P.alpha[1][j];
-    R ex[6];
-    const R* pex = nullptr;
-    if (k == ktop) {
+    R ex[6] = {0, 0, 0, 0, 0, 0};
+    const bool top = (k == ktop);
+    if (top) {
       R V[3] = {Dy[0] * d, Dy[1] * d, Dy[2] * d}, Vz[3];
 #pragma unroll
       for (int r = 0; r < 3; ++r)
         Vz[r] = s.rdzdy[(3 * r) * SS + sp] * V[0] + s.rdzdy[(3 * r + 1) * SS + sp] * V[1] + s.rdzdy[(3 * r + 2) * SS + sp] * V[2];
       stress_rows(Vz, cf(M_ZTX, p), cf(M_ZTY, p), cf(M_ZTZ, p), mu, lam, ex);
-      pex = ex;
     }
     long long i0 = i - H, k0 = k - H;
-    if (in_y1) pml_face<R>(P.f[1], i0 + (long long)a.nx * ((j - H) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, pex);
-    if (in_y2) pml_face<R>(P.f[4], i0 + (long long)a.nx * ((j - H - (a.ny - P.npml)) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, pex);
+    if (in_y1) pml_face<R>(P.f[1], i0 + (long long)a.nx * ((j - H) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, top, ex);
+    if (in_y2) pml_face<R>(P.f[4], i0 + (long long)a.nx * ((j - H - (a.ny - P.npml)) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, top, ex);
   }
   if (in_z1) {  // only for very shallow grids: bottom PML reaching the surface layers
     R Dz[9], Q[9];
@@ -436,7 +332,8 @@ __global__ __launch_bounds__(256) void cg_surface(StageArgs<R> a, const TM* __re
     for (int v = 0; v < 9; ++v) Dz[v] = mac<FB3>(W + (long long)v * N + p, S) * a.rDH;
     assemble(Dz, cf(M_ZTX, p), cf(M_ZTY, p), cf(M_ZTZ, p), mu, lam, bu, Q);
     R d = P.d[2][k], ad = d + P.alpha[2][k], bz = P.beta[2][k];
-    pml_face<R>(P.f[2], (long long)(i - H) + (long long)a.nx * ((j - H) + (long long)a.ny * (k - H)), Q, d, ad, bz, a.DT, a.stage, h, nullptr);
+    const R none[6] = {0, 0, 0, 0, 0, 0};
+    pml_face<R>(P.f[2], (long long)(i - H) + (long long)a.nx * ((j - H) + (long long)a.ny * (k - H)), Q, d, ad, bz, a.DT, a.stage, h, false, none);
   }
   R out[9];
 #pragma unroll
@@ -498,69 +395,6 @@ __global__ void cg_record(const R* W, long long N, int ns, const long long* __re
   float* o = rec + ((long long)s * nt + it) * 9;
 #pragma unroll
   for (int v = 0; v < 9; ++v) o[v] = (float)W[(long long)v * N + c];
-}
-
-// ---------------------------------------------------------------------------
-// launchers
-// ---------------------------------------------------------------------------
-template <class R, class TM, int TX, int TY>
-struct InteriorTable {
-  using Fn = void (*)(StageArgs<R>, const TM*);
-  static Fn get(int fb1, int fb2, int fb3) {
-    static const Fn t[8] = {
-        cg_interior<R, TM, -1, -1, -1, TX, TY>, cg_interior<R, TM, 1, -1, -1, TX, TY>,
-        cg_interior<R, TM, -1, 1, -1, TX, TY>,  cg_interior<R, TM, 1, 1, -1, TX, TY>,
-        cg_interior<R, TM, -1, -1, 1, TX, TY>,  cg_interior<R, TM, 1, -1, 1, TX, TY>,
-        cg_interior<R, TM, -1, 1, 1, TX, TY>,   cg_interior<R, TM, 1, 1, 1, TX, TY>};
-    return t[(fb1 > 0) + 2 * (fb2 > 0) + 4 * (fb3 > 0)];
-  }
-};
-
-template <class R, class TM>
-struct SurfaceTable {
-  using Fn = void (*)(StageArgs<R>, const TM*, SurfArgs<R>);
-  static Fn get(int fb1, int fb2, int fb3) {
-    static const Fn t[8] = {cg_surface<R, TM, -1, -1, -1>, cg_surface<R, TM, 1, -1, -1>, cg_surface<R, TM, -1, 1, -1>,
-                            cg_surface<R, TM, 1, 1, -1>,   cg_surface<R, TM, -1, -1, 1>, cg_surface<R, TM, 1, -1, 1>,
-                            cg_surface<R, TM, -1, 1, 1>,   cg_surface<R, TM, 1, 1, 1>};
-    return t[(fb1 > 0) + 2 * (fb2 > 0) + 4 * (fb3 > 0)];
-  }
-};
-
-template <class R>
-constexpr int tile_y() { return sizeof(R) == 8 ? 4 : 8; }
-
-template <class R, class TM>
-void launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, hipStream_t st) {
-  constexpr int TX = 64, TY = tile_y<R>();
-  a.ntx = (a.nx + TX - 1) / TX;
-  a.nty = (a.ny + TY - 1) / TY;
-  int nk = a.kend - a.kbeg;
-  if (nk <= 0) return;
-  a.nkc = (nk + a.kc - 1) / a.kc;
-  a.ntiles = a.ntx * a.nty * a.nkc;
-  int nb = (a.ntiles + 7) / 8 * 8;
-  auto fn = InteriorTable<R, TM, TX, TY>::get(fb1, fb2, fb3);
-  fn<<<dim3(nb), dim3(TX * TY), 0, st>>>(a, coef);
-}
-
-template <class R, class TM>
-void launch_surface(StageArgs<R> a, const TM* coef, SurfArgs<R> s, int fb1, int fb2, int fb3, hipStream_t st) {
-  dim3 grid((a.nx + 63) / 64, (a.ny + 3) / 4, 3);
-  auto fn = SurfaceTable<R, TM>::get(fb1, fb2, fb3);
-  fn<<<grid, dim3(256), 0, st>>>(a, coef, s);
-}
-
-template <class R>
-void launch_source(R* W, long long N, int np, const long long* cell, const R* rate, int srcIt, R tw, R DT, hipStream_t st) {
-  if (np <= 0) return;
-  cg_add_source<R><<<dim3((np + 127) / 128), dim3(128), 0, st>>>(W, N, np, cell, rate, srcIt, tw, DT);
-}
-
-template <class R>
-void launch_record(const R* W, long long N, int ns, const long long* cell, float* rec, int nt, int it, hipStream_t st) {
-  if (ns <= 0 || it >= nt) return;
-  cg_record<R><<<dim3((ns + 127) / 128), dim3(128), 0, st>>>(W, N, ns, cell, rec, nt, it);
 }
 
 }  // namespace cg

@@ -11,8 +11,6 @@ struct StageArgs {
   long long N;          // padded cells per field
   long long S;          // NX * NY
   int kbeg, kend;       // k range of the interior kernel [kbeg, kend)
-  int kc;               // k chunk per block
-  int ntx, nty, nkc, ntiles;  // filled by the launcher
   int stage;            // RK4 stage 0..3
   R DT, rDH;
   const R* win;         // stencil input (9 SoA fields)

@@ -1,5 +1,5 @@
 // Instantiation of the CGFDM step kernels: wavefield float, coefficients __hip_bfloat16.
-#include "cgfdm_step_impl.h"
+#include "launch_impl.h"
 
 namespace cg {
 template void launch_interior<float, __hip_bfloat16>(StageArgs<float>, const __hip_bfloat16*, int, int, int, hipStream_t);

@@ -1,5 +1,5 @@
 // Instantiation of the CGFDM step kernels: wavefield float, coefficients __half.
-#include "cgfdm_step_impl.h"
+#include "launch_impl.h"
 
 namespace cg {
 template void launch_interior<float, __half>(StageArgs<float>, const __half*, int, int, int, hipStream_t);

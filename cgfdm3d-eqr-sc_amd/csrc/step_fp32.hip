@@ -1,5 +1,5 @@
 // Instantiation of the CGFDM step kernels: wavefield float, coefficients float.
-#include "cgfdm_step_impl.h"
+#include "launch_impl.h"
 
 namespace cg {
 template void launch_interior<float, float>(StageArgs<float>, const float*, int, int, int, hipStream_t);

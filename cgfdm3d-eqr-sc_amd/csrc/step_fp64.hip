@@ -1,5 +1,5 @@
 // Instantiation of the CGFDM step kernels: wavefield double, coefficients double.
-#include "cgfdm_step_impl.h"
+#include "launch_impl.h"
 
 namespace cg {
 template void launch_interior<double, double>(StageArgs<double>, const double*, int, int, int, hipStream_t);

@@ -105,6 +105,10 @@ int cgfdm_get_ranges(cgfdm_solver* s, double* out8);
  * launches, surface ms, surface launches, step ms, steps}; out may be NULL. */
 int cgfdm_timing(cgfdm_solver* s, int enable, double* out);
 
+/* CFL inputs without a solver: [Hmin, Hmax, Vs_min, Vs_max, Vp_min, Vp_max, rho_min,
+ * rho_max] of a grid, FP64 on the GPU (calc_CFL.cpp:56-313), for choosing DT. */
+int cgfdm_grid_ranges(int nx, int ny, int nz, const float* coord, const float* medium, double* out8);
+
 /* Device bytes held by the solver. */
 long long cgfdm_device_bytes(cgfdm_solver* s);
 

@@ -4,8 +4,6 @@ Tolerances: FP64 GPU vs FP64 oracle 1e-10 (rounding order only); FP32 GPU vs
 FP32 oracle 3e-5 relative L2; FP32 GPU vs FP64 oracle 1e-4 relative L2 on the
 station seismograms (the target stated for this path).
 """
-import os
-
 import numpy as np
 import pytest
 
@@ -28,15 +26,10 @@ def solver_mod():
     return solver
 
 
-def run_pair(model, oracle_mod, solver_mod, prec, steps, kchunk=None, coef_storage=0):
-    if kchunk:
-        os.environ["CGFDM_KCHUNK"] = str(kchunk)
-    try:
-        print(f"case {model.name} {model.nx}x{model.ny}x{model.nz} npml={model.npml} fs={model.free_surface} "
-              f"prec={prec} steps={steps} kchunk={kchunk}", flush=True)
-        g = solver_mod.Solver(model, precision=prec, coef_storage=coef_storage)
-    finally:
-        os.environ.pop("CGFDM_KCHUNK", None)
+def run_pair(model, oracle_mod, solver_mod, prec, steps, coef_storage=0):
+    print(f"case {model.name} {model.nx}x{model.ny}x{model.nz} npml={model.npml} fs={model.free_surface} "
+          f"prec={prec} steps={steps}", flush=True)
+    g = solver_mod.Solver(model, precision=prec, coef_storage=coef_storage)
     o = oracle_mod.Oracle(model, prec=prec if coef_storage == 0 else 8, threads=8)
     g.step(steps)
     g.synchronize()
@@ -76,16 +69,6 @@ def test_fp32_wavefield_matches_oracle(name, oracle_mod, solver_mod):
     wg, wo = g.wavefield(), o.wavefield()
     for v in range(9):
         assert rel_l2(wg[v], wo[v]) < 3e-5, (name, v, rel_l2(wg[v], wo[v]))
-
-
-def test_k_chunking_is_exact(oracle_mod, solver_mod):
-    """Blocks that march short k chunks must give the same answer as long ones."""
-    m = make("curvilinear_pml_fs")
-    g1, _ = run_pair(m, oracle_mod, solver_mod, 8, 9, kchunk=3)
-    g2, o = run_pair(m, oracle_mod, solver_mod, 8, 9, kchunk=64)
-    for v in range(9):
-        assert rel_l2(g1.field(v), g2.field(v)) < 1e-13
-        assert rel_l2(g1.field(v), o.field(v)) < 1e-10
 
 
 def test_setup_products_match_oracle(oracle_mod, solver_mod):
