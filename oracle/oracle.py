@@ -17,8 +17,8 @@ _LIB = None
 
 def build(force=False):
     so = os.path.join(_HERE, "liboracle.so")
-    src = os.path.join(_HERE, "cgfdm_oracle.cpp")
-    if force or not os.path.exists(so) or os.path.getmtime(so) < os.path.getmtime(src):
+    srcs = [os.path.join(_HERE, f) for f in ("cgfdm_oracle.cpp", "aweno_oracle.cpp")]
+    if force or not os.path.exists(so) or os.path.getmtime(so) < max(os.path.getmtime(f) for f in srcs):
         subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "all"])
     return so
 
@@ -52,6 +52,31 @@ def lib():
         L.orc_rk_stage.argtypes = [i, i, ll, vp, vp, vp, vp]
         L.orc_step_index.restype = i
         L.orc_step_index.argtypes = [vp]
+        L.awo_create.restype = vp
+        L.awo_create.argtypes = [i, i, i, i, i, d, d, i, vp, vp]
+        L.awo_destroy.argtypes = [vp]
+        L.awo_set_source.argtypes = [vp, i, i, d, vp, vp]
+        L.awo_set_stations.argtypes = [vp, i, vp]
+        L.awo_step.argtypes = [vp, i]
+        L.awo_get_field.argtypes = [vp, i, i, vp]
+        L.awo_set_field.argtypes = [vp, i, vp]
+        L.awo_get_setup.argtypes = [vp, i, vp]
+        L.awo_rhs.argtypes = [vp]
+        L.awo_char_free_surface.argtypes = [vp]
+        L.awo_weno.restype = d
+        L.awo_weno.argtypes = [d, d, d, d, d, i]
+        L.awo_get_stations.argtypes = [vp, vp]
+        L.awo_get_pgv.argtypes = [vp, vp]
+        L.awo_get_surface_dis.argtypes = [vp, vp]
+        L.awo_step_index.restype = i
+        L.awo_step_index.argtypes = [vp]
+        L.awo_rhs_dir.argtypes = [vp, i]
+        L.awo_source_only.argtypes = [vp]
+        L.awo_observe.argtypes = [vp]
+        L.awo_sponge.argtypes = [vp]
+        L.awo_tvd_stage.argtypes = [i, i, ll, vp, vp, vp]
+        L.awo_alpha.restype = d
+        L.awo_alpha.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -165,3 +190,123 @@ def rk_stage(prec, irk, h, w, t, m):
     """Apply one low-storage RK4 stage combination in place on flat arrays."""
     L = lib()
     L.orc_rk_stage(prec, irk, h.size, _ptr(h), _ptr(w), _ptr(t), _ptr(m))
+
+
+class AwenoOracle:
+    """CPU restatement of the ME-AWENO (SCFDM) step on a `Model`.
+
+    W = (rho vx, rho vy, rho vz, exx, eyy, ezz, gyz, gxz, gxy); the model's npml
+    is ignored (the scheme absorbs with the 20-layer exponential sponge).
+    """
+
+    def __init__(self, model, prec=8, threads=None):
+        self.L = lib()
+        self.m = model
+        self.prec = prec
+        if threads:
+            self.L.orc_set_threads(int(threads))
+        self._coord = np.ascontiguousarray(model.coord, np.float32)
+        self._med = np.ascontiguousarray(model.medium, np.float32)
+        self.h = self.L.awo_create(prec, model.nx, model.ny, model.nz, 1 if model.free_surface else 0,
+                                   float(model.dh), float(model.dt), int(model.nt),
+                                   _ptr(self._coord), _ptr(self._med))
+        NZ, NY, NX = model.shape_padded
+        self.shape = (NZ, NY, NX)
+        self.N = NZ * NY * NX
+        self.ns = 0
+        if model.src_cells is not None and len(model.src_cells):
+            cells = np.ascontiguousarray(model.src_cells, np.int64)
+            rate = np.ascontiguousarray(model.src_rate, np.float32)
+            self.L.awo_set_source(self.h, len(cells), rate.shape[1], float(model.src_dt), _ptr(cells), _ptr(rate))
+        if model.stations is not None and len(model.stations):
+            st = np.ascontiguousarray(model.stations, np.int32)
+            self.L.awo_set_stations(self.h, len(st), _ptr(st))
+            self.ns = len(st)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.awo_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        self.L.awo_step(self.h, int(n))
+
+    @property
+    def it(self):
+        return self.L.awo_step_index(self.h)
+
+    @property
+    def alpha(self):
+        return self.L.awo_alpha(self.h)
+
+    def field(self, comp, which=0):
+        """which: 0 = W, 1 = h_W (last right-hand side), 2 = last face-flux buffer."""
+        out = np.empty(self.N, np.float64)
+        self.L.awo_get_field(self.h, int(which), int(comp), _ptr(out))
+        return out.reshape(self.shape)
+
+    def wavefield(self, which=0):
+        return np.stack([self.field(c, which) for c in range(9)])
+
+    def set_field(self, comp, arr):
+        a = np.ascontiguousarray(arr, np.float64).ravel()
+        self.L.awo_set_field(self.h, int(comp), _ptr(a))
+
+    def setup(self, what):
+        """CJM plane 0..18: cofactors J*xi_x..J*zt_z, J, mu, lambda, 1/rho, x_xi..z_et."""
+        out = np.empty(self.N, np.float64)
+        self.L.awo_get_setup(self.h, int(what), _ptr(out))
+        return out.reshape(self.shape)
+
+    def rhs(self):
+        """h_W = DT * (flux-difference right-hand side) of the current W."""
+        self.L.awo_rhs(self.h)
+        return self.wavefield(1)
+
+    def rhs_dir(self, d):
+        """h_W of one direction (0 = x, 1 = y, 2 = z) of the current W."""
+        self.L.awo_rhs_dir(self.h, int(d))
+        return self.wavefield(1)
+
+    def char_free_surface(self):
+        self.L.awo_char_free_surface(self.h)
+
+    def source_only(self):
+        self.L.awo_source_only(self.h)
+
+    def observe(self):
+        self.L.awo_observe(self.h)
+
+    def sponge(self):
+        self.L.awo_sponge(self.h)
+
+    def stations(self):
+        out = np.zeros((self.ns, self.m.nt, 9), np.float32)
+        if self.ns:
+            self.L.awo_get_stations(self.h, _ptr(out))
+        return out
+
+    def pgv(self):
+        out = np.zeros((self.m.ny, self.m.nx, 2), np.float32)
+        self.L.awo_get_pgv(self.h, _ptr(out))
+        return out
+
+    def surface_dis(self):
+        out = np.zeros((3, self.m.ny, self.m.nx), np.float32)
+        self.L.awo_get_surface_dis(self.h, _ptr(out))
+        return out
+
+
+def tvd_stage(prec, irk, h, w, m):
+    """One TVD-RK3 stage on float64 arrays (w and m updated in place)."""
+    for a in (h, w, m):
+        assert a.dtype == np.float64 and a.flags.c_contiguous and a.size % 9 == 0
+    lib().awo_tvd_stage(int(prec), int(irk), h.size, _ptr(h), _ptr(w), _ptr(m))
+
+
+def weno(u, prec=8):
+    """WENO5-JS value at the right face of the middle of five samples."""
+    return lib().awo_weno(*[float(x) for x in u], int(prec))
