@@ -13,12 +13,13 @@ INCLUDE = os.path.join(os.path.dirname(ROOT), "include")
 LIB = os.path.join(PKG, "libcgfdm_amd.so")
 OBJ = os.path.join(ROOT, "_build")
 
-SOURCES = ["setup.hip", "step_fp32.hip", "step_fp64.hip", "step_fp16.hip", "step_bf16.hip", "capi.cpp"]
-HEADERS = ["common.h", "kernels.h", "setup.h", "cgfdm_step_impl.h", "launch_impl.h"]  # cgfdm_zmarch_disabled.h is not built
+SOURCES = ["setup.hip", "step_fp32.hip", "step_fp64.hip", "step_fp16.hip", "step_bf16.hip", "aweno_fp32.hip",
+           "aweno_fp64.hip", "capi.cpp"]
+HEADERS = ["common.h", "kernels.h", "setup.h", "cgfdm_step_impl.h", "launch_impl.h", "aweno.h", "aweno_impl.h"]  # cgfdm_zmarch_disabled.h is not built
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
-         # no packed-FP32 instructions: unreliable on this hardware pool
+         # packed FP32 arithmetic disabled: scalar FP32 instructions only
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", CSRC, "-I", INCLUDE]
 

@@ -129,6 +129,23 @@ def layered_medium(coord, terrain, tops_m, vs, vp, rho, tolerance=1.0):
     return m
 
 
+def basin_medium(coord, terrain, center_xy, semi_axes, inside=(400.0, 1200.0, 1900.0),
+                 outside=(3464.0, 6000.0, 2700.0)):
+    """Homogeneous medium with a sharp ellipsoidal low-velocity basin: a cell
+    is inside when ((x-xc)/a)^2 + ((y-yc)/b)^2 + (depth/c)^2 <= 1, depth being
+    measured down from the terrain surface above it."""
+    x = coord[0].astype(np.float64)
+    y = coord[1].astype(np.float64)
+    depth = terrain.astype(np.float64)[None, :, :] - coord[2].astype(np.float64)
+    a, b, c = semi_axes
+    r = ((x - center_xy[0]) / a) ** 2 + ((y - center_xy[1]) / b) ** 2 + (np.maximum(depth, 0.0) / c) ** 2
+    m = np.empty((3,) + x.shape, np.float32)
+    ins = r <= 1.0
+    for q in range(3):
+        m[q] = np.where(ins, inside[q], outside[q])
+    return m
+
+
 def gaussian_random_field(shape, dh, corr_len, sigma, seed, clip=3.0):
     """Zero-mean Gaussian field with Gaussian autocorrelation (FFT-filtered noise)."""
     import scipy.fft as sfft
@@ -276,6 +293,44 @@ def config2(nt=3000, nx=512, ny=512, nz=256, dh=100.0, npml=20, seed=1002, nhill
     point_source(m, i, j, k, 30.0, 60.0, 90.0, 1e16, 2.0)
     nside = 20 if min(nx, ny) - 2 * npml >= 20 else max(2, min(nx, ny) - 2 * npml)
     m.stations = surface_station_grid(nx, ny, nz, npml, nside)
+    return m
+
+
+def config3(nt=4000, nx=512, ny=512, nz=256, dh=100.0, seed=1002, nhills=8, hmin_fn=None):
+    """ME-AWENO: the 512x512x256 terrain grid of config2 with a sharp
+    low-velocity basin (Vs 400, Vp 1200, rho 1900 inside an ellipsoid of
+    20 x 10 x 2 km extent under the centre of the surface; Vs 3464, Vp 6000,
+    rho 2700 outside), point moment source at 5 km depth, 400 surface stations.
+    Absorption is the scheme's 20-layer exponential sponge."""
+    depth = nz * dh
+    terrain = gaussian_hills(nx, ny, dh, 20, nhills, seed)
+    coord = terrain_coord(nx, ny, nz, dh, terrain, depth)
+    med = basin_medium(coord, terrain, ((nx // 2) * dh, (ny // 2) * dh), (10000.0, 5000.0, 2000.0))
+    hmin = (hmin_fn or (lambda c, md: hmin_point_plane(c)))(coord, med)
+    dt = 0.5 * hmin / float(med[1].max())
+    m = Model(nx, ny, nz, dh, dt, nt, 0, coord, med, True, name="config3", terrain=terrain)
+    depth_cells = int(round(5000.0 / dh))
+    i, j, k = nx // 2 + HALO, ny // 2 + HALO, nz - 1 - depth_cells + HALO
+    point_source(m, i, j, k, 30.0, 60.0, 90.0, 1e16, 2.0)
+    nside = 20 if min(nx, ny) - 40 >= 20 else max(2, min(nx, ny) - 40)
+    m.stations = surface_station_grid(nx, ny, nz, 20, nside)
+    return m
+
+
+def small_basin_config(nx=48, ny=44, nz=20, dh=100.0, nt=12, seed=5, hills=2):
+    """Small terrain grid with a sharp low-velocity basin (Vs 400 / Vp 1200 /
+    rho 1900 in an ellipsoid 1.6 x 1.2 x 0.6 km under the surface centre)."""
+    depth = nz * dh
+    terrain = gaussian_hills(nx, ny, dh, 4, hills, seed, hmin=100.0, hmax=300.0, smin=300.0, smax=900.0)
+    coord = terrain_coord(nx, ny, nz, dh, terrain, depth)
+    med = basin_medium(coord, terrain, ((nx // 2) * dh, (ny // 2) * dh), (800.0, 600.0, 600.0))
+    hmin = hmin_point_plane(coord)
+    dt = 0.5 * hmin / float(med[1].max())
+    m = Model(nx, ny, nz, dh, dt, nt, 0, coord, med, True, name="small_basin", terrain=terrain)
+    i, j, k = nx // 2 + HALO + 3, ny // 2 + HALO - 2, nz // 2 + HALO - 2
+    point_source(m, i, j, k, 30.0, 60.0, 90.0, 1e16, 8.0)
+    m.stations = np.array([[nx // 2 + HALO, ny // 2 + HALO, nz - 1 + HALO], [HALO + 1, HALO + 2, nz - 1 + HALO],
+                           [i + 2, j - 3, k], [nx + HALO - 1, ny + HALO - 1, HALO]], np.int32)
     return m
 
 

@@ -99,12 +99,16 @@ def grid_ranges(nx, ny, nz, coord, medium):
 class Solver:
     """One single-GPU domain advanced by the HIP kernels.
 
-    precision: 4 (FP32 wavefield) or 8 (FP64); coef_storage: 0 native, 1 fp16, 2 bf16.
+    scheme: SCHEME_CGFDM (MacCormack + RK4 + CFS-PML, W = velocity/stress) or
+    SCHEME_AWENO (WENO5 alternative flux + TVD-RK3 + sponge, W = momentum/strain;
+    the model's npml is not used).
+    precision: 4 (FP32 wavefield) or 8 (FP64); coef_storage: 0 native, 1 fp16, 2 bf16 (CGFDM).
     """
 
     def __init__(self, model: Model, precision=4, coef_storage=0, scheme=SCHEME_CGFDM, stream=None):
         self.L = lib()
         self.m = model
+        self.scheme = scheme
         desc = _Desc(model.nx, model.ny, model.nz, model.npml, 1 if model.free_surface else 0, scheme,
                      precision, coef_storage, model.nt, float(model.dh), float(model.dt))
         coord = np.ascontiguousarray(model.coord, np.float32)
@@ -187,8 +191,12 @@ class Solver:
         _check(self.L.cgfdm_set_field(self.h, int(comp), _ptr(a)))
 
     def setup(self, what):
+        """Setup product `what`. CGFDM: 0..8 metrics, 9 J, 10..12 mu/lambda/(1/rho),
+        20..28 / 30..38 surface matrices, 40.. PML profiles. ME-AWENO: the 19
+        coefficient planes (0..8 cofactors J*xi_x.., 9 J, 10..12 medium,
+        13..18 x_xi, x_eta)."""
         NZ, NY, NX = self.shape
-        if what < 13:
+        if what < 13 or (self.scheme == SCHEME_AWENO and what < 19):
             n, shp = self.N, self.shape
         elif what < 40:
             n, shp = NY * NX, (NY, NX)

@@ -1,0 +1,471 @@
+// ME-AWENO (SCFDM) step kernels for gfx950: WENO5 alternative-flux face
+// fluxes, flux-difference update fused with TVD-RK3, characteristic free
+// surface, exponential sponge, strain-rate source and observers.
+//
+// Layout: conservative state W = (rho vx, rho vy, rho vz, exx, eyy, ezz, gyz,
+// gxz, gxy) as 9 SoA planes over the padded grid (x fastest, 3 ghost layers).
+// Coefficients: 19 SoA planes (CJM order of the SCFDM build, CJM.cpp:107-147):
+// 0..8 cofactors J*xi_x .. J*zt_z, 9 J, 10 mu, 11 lambda, 12 1/rho,
+// 13..15 x_xi, 16..18 x_eta.
+#pragma once
+#include "aweno.h"
+
+namespace aw {
+
+using cg::H;
+
+// WENO5-JS value at x_{i+1/2} from u1..u5 = u(i-2..i+2)
+// (alternative_flux_FD.cpp:43-59). The weights are d_k / (beta_k + 1e-6)^2 with
+// d = (5/16, 10/16, 1/16), evaluated in the working precision.
+template <class R>
+__device__ __forceinline__ R weno5(R u1, R u2, R u3, R u4, R u5) {
+  const R third = R(0.3333333333);
+  R b1 = third * (R(10) * u3 * u3 - R(31) * u3 * u4 + R(25) * u4 * u4 + R(11) * u3 * u5 - R(19) * u4 * u5 + R(4) * u5 * u5);
+  R b2 = third * (R(4) * u2 * u2 - R(13) * u2 * u3 + R(13) * u3 * u3 + R(5) * u2 * u4 - R(13) * u3 * u4 + R(4) * u4 * u4);
+  R b3 = third * (R(4) * u1 * u1 - R(19) * u1 * u2 + R(25) * u2 * u2 + R(11) * u1 * u3 - R(31) * u2 * u3 + R(10) * u3 * u3);
+  const R eps = R(1e-6);
+  R q1 = b1 + eps, q2 = b2 + eps, q3 = b3 + eps;
+  R w1 = R(0.3125) / (q1 * q1);
+  R w2 = R(0.625) / (q2 * q2);
+  R w3 = R(0.0625) / (q3 * q3);
+  R tot = R(1) / (w1 + w2 + w3);
+  return (w1 * tot) * (R(0.375) * u3 + R(0.75) * u4 - R(0.125) * u5) +
+         (w2 * tot) * (R(-0.125) * u2 + R(0.75) * u3 + R(0.375) * u4) +
+         (w3 * tot) * (R(0.375) * u1 - R(1.25) * u2 + R(1.875) * u3);
+}
+
+// Conservative elastic flux through a face of cofactor normal (n1, n2, n3)
+// (alternative_flux_FD.cpp:140-158).
+template <class R>
+__device__ __forceinline__ void phys_flux(const R (&u)[9], R n1, R n2, R n3, R mu, R lam, R b, R (&f)[9]) {
+  const R l2m = lam + R(2) * mu;
+  f[0] = -(n1 * (lam * u[4] + lam * u[5] + u[3] * l2m) + mu * n2 * u[8] + mu * n3 * u[7]);
+  f[1] = -(n2 * (lam * u[3] + lam * u[5] + u[4] * l2m) + mu * n1 * u[8] + mu * n3 * u[6]);
+  f[2] = -(n3 * (lam * u[3] + lam * u[4] + u[5] * l2m) + mu * n1 * u[7] + mu * n2 * u[6]);
+  f[3] = -((n1 * u[0]) * b);
+  f[4] = -((n2 * u[1]) * b);
+  f[5] = -((n3 * u[2]) * b);
+  f[6] = -((n2 * u[2]) * b + (n3 * u[1]) * b);
+  f[7] = -((n1 * u[2]) * b + (n3 * u[0]) * b);
+  f[8] = -((n1 * u[1]) * b + (n2 * u[0]) * b);
+}
+
+// Lax-Friedrichs face flux of direction d at face q+1/2, stored at cell q
+// (wave_deriv_alternative_flux_FD_{x,y,z}).
+template <class R>
+__device__ __forceinline__ void face_flux(const StageArgs<R>& a, long long p, long long s, int d) {
+  const long long N = a.N;
+  const R n1 = a.cjm[(3 * d + 0) * N + p], n2 = a.cjm[(3 * d + 1) * N + p], n3 = a.cjm[(3 * d + 2) * N + p];
+  const R mu = a.cjm[C_MU * N + p], lam = a.cjm[C_LAM * N + p], b = a.cjm[C_B * N + p];
+  R um[9], up[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    const R* w = a.win + v * N;
+    const R wm2 = w[p - 2 * s], wm1 = w[p - s], w0 = w[p], wp1 = w[p + s], wp2 = w[p + 2 * s], wp3 = w[p + 3 * s];
+    um[v] = weno5<R>(wm2, wm1, w0, wp1, wp2);
+    up[v] = weno5<R>(wp3, wp2, wp1, w0, wm1);
+  }
+  R fm[9], fp[9];
+  phys_flux<R>(um, n1, n2, n3, mu, lam, b, fm);
+  phys_flux<R>(up, n1, n2, n3, mu, lam, b, fp);
+  R* F = a.F + (long long)(9 * d) * N;
+#pragma unroll
+  for (int v = 0; v < 9; ++v) F[v * N + p] = R(0.5) * (fp[v] + fm[v] - a.alpha * (up[v] - um[v]));
+}
+
+// Face fluxes of the three directions. Thread (i, j, k) over [2, n-3)^3
+// computes the x face when j, k are interior, the y face when i, k are, the
+// z face when i, j are: exactly the faces the reference writes, so every other
+// entry of F keeps the zero it was allocated with.
+template <class R>
+__global__ void __launch_bounds__(256) aw_faces(StageArgs<R> a) {
+  const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k = 2 + blockIdx.z;
+  if (i >= a.NX - 3 || j >= a.NY - 3) return;
+  const long long p = (long long)i + (long long)a.NX * j + (long long)k * a.S;
+  const bool ii = i >= H, ji = j >= H, ki = k >= H;
+  if (ji && ki) face_flux<R>(a, p, 1, 0);
+  if (ii && ki) face_flux<R>(a, p, a.NX, 1);
+  if (ii && ji) face_flux<R>(a, p, a.S, 2);
+}
+
+template <class R>
+__device__ __forceinline__ R d2c(R u1, R u2, R u3, R u4, R u5) {
+  return R(1.0 / 12.0) * (-u1 + R(16) * u2 - R(30) * u3 + R(16) * u4 - u5);
+}
+template <class R>
+__device__ __forceinline__ R d4c(R u1, R u2, R u3, R u4, R u5) {
+  return u1 - R(4) * u2 + R(6) * u3 - R(4) * u4 + u5;
+}
+template <class R>
+__device__ __forceinline__ R d2f(R u1, R u2, R u3, R u4, R u5, R u6) {
+  return R(1.0 / 48.0) * (R(-5) * u1 + R(39) * u2 - R(34) * u3 - R(34) * u4 + R(39) * u5 - R(5) * u6);
+}
+template <class R>
+__device__ __forceinline__ R d4f(R u1, R u2, R u3, R u4, R u5, R u6) {
+  return R(0.5) * (u1 - R(3) * u2 + R(2) * u3 + R(2) * u4 - R(3) * u5 + u6);
+}
+
+// TVD-RK3 combination (wave_rk_tvd.cpp:18-78); stages 1 and 2 in FP64 as the
+// reference's double literals make them.
+template <class R>
+__device__ __forceinline__ R tvd(int stage, R m, R w, R h) {
+  if (stage == 0) return w + h;
+  if (stage == 1) return R(0.75 * (double)m + 0.25 * (double)w + 0.25 * (double)h);
+  return R((1.0 / 3.0) * (double)m + (2.0 / 3.0) * (double)w + (2.0 / 3.0) * (double)h);
+}
+
+// Flux difference with the alternative-flux correction in all three
+// directions (cal_du_{x,y,z}, alternative_flux_FD.cpp:1216-1435), fused with
+// the TVD-RK3 update. One thread per padded cell: outside the interior the
+// right-hand side is zero (the reference's TVD kernels run over the whole
+// padded arrays). Cells of the three layers under the top are left to
+// aw_update_top.
+template <class R>
+__global__ void __launch_bounds__(256) aw_update(StageArgs<R> a) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k = blockIdx.z;
+  if (i >= a.NX || j >= a.NY) return;
+  const long long N = a.N;
+  const long long p = (long long)i + (long long)a.NX * j + (long long)k * a.S;
+  const bool interior = i >= H && i < a.NX - H && j >= H && j < a.NY - H && k >= H && k < a.NZ - H;
+  const int ktop3 = a.NZ - H - 3;
+  if (interior && k >= ktop3) return;
+  R h[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) h[v] = R(0);
+  if (interior) {
+    const R jac = R(1.0 / (double)a.cjm[C_J * N + p]);
+    const R cd = a.DT * a.rDH * jac;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const long long s = d == 0 ? 1 : d == 1 ? (long long)a.NX : a.S;
+      const R* F = a.F + (long long)(9 * d) * N;
+#pragma unroll
+      for (int v = 0; v < 9; ++v) {
+        const R* f = F + v * N;
+        const R f_3 = f[p - 3 * s], f_2 = f[p - 2 * s], f_1 = f[p - s], f0 = f[p], f1 = f[p + s], f2 = f[p + 2 * s];
+        const R fr = f0 - R(1.0 / 24.0) * d2c<R>(f_2, f_1, f0, f1, f2) + R(7.0 / 5760.0) * d4c<R>(f_2, f_1, f0, f1, f2);
+        const R fl = f_1 - R(1.0 / 24.0) * d2c<R>(f_3, f_2, f_1, f0, f1) + R(7.0 / 5760.0) * d4c<R>(f_3, f_2, f_1, f0, f1);
+        h[v] += cd * (-(fr - fl));
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 9; ++v) a.wout[v * N + p] = tvd<R>(a.stage, a.wm[v * N + p], a.win[v * N + p], h[v]);
+}
+
+// The three interior layers under the top: the corrections use 6-point
+// formulas on the physical fluxes of the 7 cells around the cell
+// (alternative_flux_FD.cpp:1257-1278), in every direction.
+template <class R>
+__global__ void __launch_bounds__(256) aw_update_top(StageArgs<R> a, int nx, int ny) {
+  const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k = a.NZ - H - 3 + blockIdx.z;
+  if (i >= H + nx || j >= H + ny) return;
+  const long long N = a.N;
+  const long long p = (long long)i + (long long)a.NX * j + (long long)k * a.S;
+  const R jac = R(1.0 / (double)a.cjm[C_J * N + p]);
+  const R cd = a.DT * a.rDH * jac;
+  R h[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) h[v] = R(0);
+  for (int d = 0; d < 3; ++d) {
+    const long long s = d == 0 ? 1 : d == 1 ? (long long)a.NX : a.S;
+    R hu[7][9];
+#pragma unroll
+    for (int n = 0; n < 7; ++n) {
+      const long long q = p + (n - 3) * s;
+      R u[9];
+#pragma unroll
+      for (int v = 0; v < 9; ++v) u[v] = a.win[v * N + q];
+      phys_flux<R>(u, a.cjm[(3 * d) * N + q], a.cjm[(3 * d + 1) * N + q], a.cjm[(3 * d + 2) * N + q],
+                   a.cjm[C_MU * N + q], a.cjm[C_LAM * N + q], a.cjm[C_B * N + q], hu[n]);
+    }
+    const R* F = a.F + (long long)(9 * d) * N;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+      const R fr = F[v * N + p] - R(1.0 / 24.0) * d2f<R>(hu[1][v], hu[2][v], hu[3][v], hu[4][v], hu[5][v], hu[6][v]) +
+                   R(7.0 / 5760.0) * d4f<R>(hu[1][v], hu[2][v], hu[3][v], hu[4][v], hu[5][v], hu[6][v]);
+      const R fl = F[v * N + p - s] - R(1.0 / 24.0) * d2f<R>(hu[0][v], hu[1][v], hu[2][v], hu[3][v], hu[4][v], hu[5][v]) +
+                   R(7.0 / 5760.0) * d4f<R>(hu[0][v], hu[1][v], hu[2][v], hu[3][v], hu[4][v], hu[5][v]);
+      h[v] += cd * (-(fr - fl));
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 9; ++v) a.wout[v * N + p] = tvd<R>(a.stage, a.wm[v * N + p], a.win[v * N + p], h[v]);
+}
+
+// Characteristic free surface on the top layer and cubic extrapolation into
+// the three ghost layers (freeSurface.cpp:41-194). The stress tensor and the
+// velocity are expressed in the frame B = [n s t] of the surface normal and
+// the tangents x_xi, x_eta: frame components B^-1 S B^-T and B^-1 v, the rows
+// of det(B) B^-1 being (s x t, t x n, n x s).
+template <class R>
+__global__ void __launch_bounds__(256) aw_char_surface(R* __restrict__ W, const R* __restrict__ cjm, int nx, int ny,
+                                                       int NX, int NZ, long long N, long long S) {
+  const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= H + nx || j >= H + ny) return;
+  const int k = NZ - H - 1;
+  const long long p = (long long)i + (long long)NX * j + (long long)k * S;
+  const R mu = cjm[C_MU * N + p], lam = cjm[C_LAM * N + p], b = cjm[C_B * N + p];
+  const R rho = R(1) / b;
+  const R vs = sqrt(mu * b), vp = sqrt((lam + R(2) * mu) * b);
+  const R J = cjm[C_J * N + p];
+  R B[3][3];
+  {
+    R n0 = cjm[6 * N + p] / J, n1 = cjm[7 * N + p] / J, n2 = cjm[8 * N + p] / J;
+    R nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    R s0 = cjm[(C_TXI + 0) * N + p], s1 = cjm[(C_TXI + 1) * N + p], s2 = cjm[(C_TXI + 2) * N + p];
+    R sn = sqrt(s0 * s0 + s1 * s1 + s2 * s2);
+    R t0 = cjm[(C_TET + 0) * N + p], t1 = cjm[(C_TET + 1) * N + p], t2 = cjm[(C_TET + 2) * N + p];
+    R tn = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+    B[0][0] = n0 / nn; B[0][1] = n1 / nn; B[0][2] = n2 / nn;
+    B[1][0] = s0 / sn; B[1][1] = s1 / sn; B[1][2] = s2 / sn;
+    B[2][0] = t0 / tn; B[2][1] = t1 / tn; B[2][2] = t2 / tn;
+  }
+  R Rw[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int u = (r + 1) % 3, v = (r + 2) % 3;
+    Rw[r][0] = B[u][1] * B[v][2] - B[u][2] * B[v][1];
+    Rw[r][1] = B[u][2] * B[v][0] - B[u][0] * B[v][2];
+    Rw[r][2] = B[u][0] * B[v][1] - B[u][1] * B[v][0];
+  }
+  const R det = B[0][0] * Rw[0][0] + B[0][1] * Rw[0][1] + B[0][2] * Rw[0][2];
+  R w[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) w[q] = W[q * N + p];
+  const R l2m = lam + R(2) * mu;
+  R Sg[3][3];
+  Sg[0][0] = lam * w[4] + lam * w[5] + w[3] * l2m;
+  Sg[1][1] = lam * w[3] + lam * w[5] + w[4] * l2m;
+  Sg[2][2] = lam * w[3] + lam * w[4] + w[5] * l2m;
+  Sg[0][1] = Sg[1][0] = mu * w[8];
+  Sg[0][2] = Sg[2][0] = mu * w[7];
+  Sg[1][2] = Sg[2][1] = mu * w[6];
+  R vel[3] = {w[0] * b, w[1] * b, w[2] * b};
+  R Sf[3][3], vf[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    R RS[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) RS[q] = Rw[r][0] * Sg[0][q] + Rw[r][1] * Sg[1][q] + Rw[r][2] * Sg[2][q];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Sf[r][c] = (RS[0] * Rw[c][0] + RS[1] * Rw[c][1] + RS[2] * Rw[c][2]) / (det * det);
+    vf[r] = (Rw[r][0] * vel[0] + Rw[r][1] * vel[1] + Rw[r][2] * vel[2]) / det;
+  }
+  // traction-free characteristic condition (freeSurface.cpp:145-153)
+  Sf[1][1] -= Sf[0][0] * lam / l2m;
+  Sf[2][2] -= Sf[0][0] * lam / l2m;
+  vf[1] -= Sf[0][1] / (vs * rho);
+  vf[2] -= Sf[0][2] / (vs * rho);
+  vf[0] -= Sf[0][0] / (vp * rho);
+  Sf[0][0] = R(0);
+  Sf[0][1] = Sf[1][0] = R(0);
+  Sf[0][2] = Sf[2][0] = R(0);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      R acc = R(0);
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc += Sf[u][c] * B[u][r] * B[c][q];
+      Sg[r][q] = acc;
+    }
+    vel[r] = B[0][r] * vf[0] + B[1][r] * vf[1] + B[2][r] * vf[2];
+  }
+  const R cc = R(1) / (R(2) * mu * (R(3) * lam + R(2) * mu));
+  R o[9];
+  o[0] = vel[0] * rho;
+  o[1] = vel[1] * rho;
+  o[2] = vel[2] * rho;
+  o[3] = (R(2) * lam * Sg[0][0] - lam * Sg[1][1] - lam * Sg[2][2] + R(2) * mu * Sg[0][0]) * cc;
+  o[4] = -(lam * Sg[0][0] - R(2) * lam * Sg[1][1] + lam * Sg[2][2] - R(2) * mu * Sg[1][1]) * cc;
+  o[5] = -(lam * Sg[0][0] + lam * Sg[1][1] - R(2) * lam * Sg[2][2] - R(2) * mu * Sg[2][2]) * cc;
+  const R rm = R(1) / mu;
+  o[6] = Sg[1][2] * rm;
+  o[7] = Sg[0][2] * rm;
+  o[8] = Sg[0][1] * rm;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    R* f = W + q * N;
+    R c0 = o[q], c1 = f[p - S], c2 = f[p - 2 * S], c3 = f[p - 3 * S];
+    f[p] = c0;
+#pragma unroll
+    for (int g = 1; g <= H; ++g) {
+      const R e = R(4) * c0 - R(6) * c1 + R(4) * c2 - c3;
+      f[p + g * S] = e;
+      c3 = c2;
+      c2 = c1;
+      c1 = c0;
+      c0 = e;
+    }
+  }
+}
+
+// Exponential sponge (exp_decay_absorb.cpp:20-106): 20 layers, alpha 0.015,
+// factors of the sides a cell lies in applied one after another (x-low,
+// x-high, y-low, y-high, z-low, z-high without free surface).
+template <class R>
+__global__ void __launch_bounds__(256) aw_sponge(R* __restrict__ W, int nx, int ny, int nz, int NX, long long N,
+                                                 long long S, int fs) {
+  const int I = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int Jj = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int K = blockIdx.z;
+  if (I >= nx || Jj >= ny) return;
+  const int L = 20;
+  int pv[6];
+  int n = 0;
+  if (I <= L) pv[n++] = L - I;
+  if (I >= nx - L) pv[n++] = L - (nx - I);
+  if (Jj <= L) pv[n++] = L - Jj;
+  if (Jj >= ny - L) pv[n++] = L - (ny - Jj);
+  if (K <= L) pv[n++] = L - K;
+  if (!fs && K >= nz - L) pv[n++] = L - (nz - K);
+  if (n == 0) return;
+  const long long p = (long long)(I + H) + (long long)NX * (Jj + H) + (long long)(K + H) * S;
+  const R al = R(0.015);
+  R w[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) w[v] = W[v * N + p];
+  for (int q = 0; q < n; ++q) {
+    const R x = al * R(pv[q]);
+    const R g = exp(-x * x);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) w[v] *= g;
+  }
+#pragma unroll
+  for (int v = 0; v < 9; ++v) W[v * N + p] = w[v];
+}
+
+// Strain-rate source (addMoment.cpp:192-239): the interpolated moment rate
+// (already scaled by mu / (J DH^3)) is turned into strain rates by the
+// compliance of the cell and subtracted from W[3..8] once per step.
+template <class R>
+__global__ void aw_source(R* __restrict__ W, const R* __restrict__ cjm, long long N, int np,
+                          const long long* __restrict__ cell, const R* __restrict__ rate, int srcIt, R tw, R DT) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const long long c = cell[q];
+  R s[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    const R m0 = rate[((long long)srcIt * np + q) * 6 + m];
+    const R m1 = rate[((long long)(srcIt + 1) * np + q) * 6 + m];
+    s[m] = (m1 - m0) * tw + m0;
+  }
+  const R mu = cjm[C_MU * N + c], lam = cjm[C_LAM * N + c];
+  const R den = R(2) * (mu * mu) + R(3) * lam * mu;
+  R e[6];
+  e[0] = (s[0] * (lam + mu)) / den - (lam * s[1]) / (R(2) * den) - (lam * s[2]) / (R(2) * den);
+  e[1] = (s[1] * (lam + mu)) / den - (lam * s[0]) / (R(2) * den) - (lam * s[2]) / (R(2) * den);
+  e[2] = (s[2] * (lam + mu)) / den - (lam * s[0]) / (R(2) * den) - (lam * s[1]) / (R(2) * den);
+  e[3] = s[5] / mu;
+  e[4] = s[4] / mu;
+  e[5] = s[3] / mu;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) W[(3 + m) * N + c] -= e[m] * DT;
+}
+
+// Surface observers of the SCFDM build: PGV (PGV.cpp:62-175) and the surface
+// displacement (solve_displacement.cpp:25-45) from v = (rho v) / rho.
+template <class R>
+__global__ void __launch_bounds__(256) aw_surface_obs(const R* __restrict__ W, const R* __restrict__ cjm, float* pgv,
+                                                      R* dis, int nx, int ny, int NX, int NZ, long long N, long long S,
+                                                      R DT) {
+  const int I = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int Jj = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (I >= nx || Jj >= ny) return;
+  const long long p = (long long)(I + H) + (long long)NX * (Jj + H) + (long long)(NZ - H - 1) * S;
+  const R b = cjm[C_B * N + p];
+  const R vx = W[p] * b, vy = W[N + p] * b, vz = W[2 * N + p] * b;
+  const long long q = (long long)I + (long long)nx * Jj;
+  const long long qd = (long long)(I + H) + (long long)NX * (Jj + H);
+  dis[qd] += vx * DT;
+  dis[S + qd] += vy * DT;
+  dis[2 * S + qd] += vz * DT;
+  const float fx = (float)vx, fy = (float)vy, fz = (float)vz;
+  const float vh = sqrtf(fx * fx + fy * fy), va = sqrtf(fx * fx + fy * fy + fz * fz);
+  if (pgv[2 * q] < vh) pgv[2 * q] = vh;
+  if (pgv[2 * q + 1] < va) pgv[2 * q + 1] = va;
+}
+
+// Station record in physical variables (station.cpp:303-380, SCFDM branch):
+// (vx, vy, vz, sxx, syy, szz, sxy, sxz, syz).
+template <class R>
+__global__ void aw_record(const R* __restrict__ W, const R* __restrict__ cjm, long long N, int ns,
+                          const long long* __restrict__ cell, float* rec, int nt, int it) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const long long p = cell[s];
+  R u[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) u[v] = W[v * N + p];
+  const R mu = cjm[C_MU * N + p], lam = cjm[C_LAM * N + p], b = cjm[C_B * N + p];
+  float* o = rec + ((long long)s * nt + it) * 9;
+  o[0] = (float)(u[0] * b);
+  o[1] = (float)(u[1] * b);
+  o[2] = (float)(u[2] * b);
+  o[3] = (float)(lam * u[4] + lam * u[5] + u[3] * (lam + R(2) * mu));
+  o[4] = (float)(lam * u[3] + lam * u[5] + u[4] * (lam + R(2) * mu));
+  o[5] = (float)(lam * u[3] + lam * u[4] + u[5] * (lam + R(2) * mu));
+  o[6] = (float)(mu * u[8]);
+  o[7] = (float)(mu * u[7]);
+  o[8] = (float)(mu * u[6]);
+}
+
+// --------------------------------------------------------------- launchers
+
+template <class R>
+void launch_faces(const StageArgs<R>& a, hipStream_t st) {
+  const dim3 grid((a.NX - 5 + 63) / 64, (a.NY - 5 + 3) / 4, a.NZ - 5), block(256);
+  aw_faces<R><<<grid, block, 0, st>>>(a);
+}
+
+template <class R>
+void launch_update(const StageArgs<R>& a, int nx, int ny, hipStream_t st) {
+  const dim3 grid((a.NX + 63) / 64, (a.NY + 3) / 4, a.NZ), block(256);
+  aw_update<R><<<grid, block, 0, st>>>(a);
+  const dim3 gtop((nx + 63) / 64, (ny + 3) / 4, 3);
+  aw_update_top<R><<<gtop, block, 0, st>>>(a, nx, ny);
+}
+
+template <class R>
+void launch_char_surface(R* W, const R* cjm, int nx, int ny, int NX, int NZ, long long N, long long S, hipStream_t st) {
+  aw_char_surface<R><<<dim3((nx + 63) / 64, (ny + 3) / 4, 1), dim3(256), 0, st>>>(W, cjm, nx, ny, NX, NZ, N, S);
+}
+
+template <class R>
+void launch_sponge(R* W, int nx, int ny, int nz, int NX, long long N, long long S, int fs, hipStream_t st) {
+  aw_sponge<R><<<dim3((nx + 63) / 64, (ny + 3) / 4, nz), dim3(256), 0, st>>>(W, nx, ny, nz, NX, N, S, fs);
+}
+
+template <class R>
+void launch_source(R* W, const R* cjm, long long N, int np, const long long* cell, const R* rate, int srcIt, R tw, R DT,
+                   hipStream_t st) {
+  if (np <= 0) return;
+  aw_source<R><<<dim3((np + 127) / 128), dim3(128), 0, st>>>(W, cjm, N, np, cell, rate, srcIt, tw, DT);
+}
+
+template <class R>
+void launch_surface_obs(const R* W, const R* cjm, float* pgv, R* dis, int nx, int ny, int NX, int NZ, long long N,
+                        long long S, R DT, hipStream_t st) {
+  aw_surface_obs<R><<<dim3((nx + 63) / 64, (ny + 3) / 4, 1), dim3(256), 0, st>>>(W, cjm, pgv, dis, nx, ny, NX, NZ, N,
+                                                                                  S, DT);
+}
+
+template <class R>
+void launch_record(const R* W, const R* cjm, long long N, int ns, const long long* cell, float* rec, int nt, int it,
+                   hipStream_t st) {
+  if (ns <= 0 || it >= nt) return;
+  aw_record<R><<<dim3((ns + 127) / 128), dim3(128), 0, st>>>(W, cjm, N, ns, cell, rec, nt, it);
+}
+
+}  // namespace aw

@@ -13,6 +13,7 @@
 
 #include "cgfdm_amd.h"
 #include "common.h"
+#include "aweno.h"
 #include "kernels.h"
 #include "setup.h"
 
@@ -39,7 +40,13 @@ const int FBTAB[8][3] = {{-1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, -1, 1},
 constexpr double REF_PI = 3.141592657;  // reference macro.h:70, enters the PML alpha profile
 
 struct SolverBase {
-  virtual ~SolverBase() {}
+  virtual ~SolverBase() {
+    for (auto& e : ev) {
+      hipEventDestroy(e.a);
+      hipEventDestroy(e.b);
+    }
+    for (void* p : allocs) hipFree(p);
+  }
   virtual int init(const cgfdm_desc& d, const float* coord, const float* medium) = 0;
   virtual int set_source(int np, int nt, double dt, const long long* cell, const float* rate) = 0;
   virtual int set_stations(int ns, const int* ijk) = 0;
@@ -50,60 +57,13 @@ struct SolverBase {
   virtual int get_field(int comp, double* out) = 0;
   virtual int set_field(int comp, const double* in) = 0;
   virtual int get_setup(int what, double* out) = 0;
-  virtual int timing(int enable, double* out) = 0;
   hipStream_t stream = nullptr;
   int it = 0;
   double ranges[8] = {0};
   long long bytes = 0;
-};
 
-template <class T>
-float to_float(T v) { return (float)v; }
-template <>
-float to_float<__half>(__half v) { return __half2float(v); }
-template <>
-float to_float<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
-
-template <class R, class TM>
-struct Solver : SolverBase {
-  cgfdm_desc d{};
-  int NX = 0, NY = 0, NZ = 0;
-  long long N = 0, S = 0;
-  double Z = 1.0;  // stress carried as T / Z
+  // device allocations owned by the solver, zero-filled
   std::vector<void*> allocs;
-  R *A = nullptr, *B = nullptr, *C = nullptr, *T = nullptr;
-  TM* coef = nullptr;
-  R* jac = nullptr;
-  R *rdzdx = nullptr, *rdzdy = nullptr;
-  R* pml_par = nullptr;  // alpha, beta, d for x (NX each), then y, then z
-  cg::PmlArgs<R> pml{};
-  float* pgv = nullptr;
-  R* dis = nullptr;
-  int ns = 0;
-  long long* st_cell = nullptr;
-  float* st_rec = nullptr;
-  int src_np = 0, src_nt = 0;
-  double src_dt = 0;
-  long long* src_cell = nullptr;
-  R* src_rate = nullptr;
-  // timing
-  bool timing_on = false;
-  struct EvPair {
-    hipEvent_t a, b;
-    int kind;  // 0 interior kernel, 1 surface kernel, 2 whole step
-  };
-  std::vector<EvPair> ev;
-  size_t ev_used = 0;
-  double tacc[6] = {0};
-
-  ~Solver() override {
-    for (auto& e : ev) {
-      hipEventDestroy(e.a);
-      hipEventDestroy(e.b);
-    }
-    for (void* p : allocs) hipFree(p);
-  }
-
   template <class X>
   int dalloc(X** p, long long count) {
     size_t nb = (size_t)count * sizeof(X);
@@ -127,6 +87,100 @@ struct Solver : SolverBase {
       }
   }
 
+  // kernel timing with HIP events on the solver's stream
+  bool timing_on = false;
+  struct EvPair {
+    hipEvent_t a, b;
+    int kind;  // 0 main stencil kernel, 1 second kernel, 2 whole step
+  };
+  std::vector<EvPair> ev;
+  size_t ev_used = 0;
+  double tacc[6] = {0};
+  size_t next_pair(int kind) {
+    if (ev_used >= ev.size()) {
+      EvPair e{};
+      hipEventCreate(&e.a);
+      hipEventCreate(&e.b);
+      ev.push_back(e);
+    }
+    ev[ev_used].kind = kind;
+    return ev_used++;
+  }
+  int timing(int enable, double* out) {
+    if (timing_on && !enable) {
+      HIPCHK(hipStreamSynchronize(stream));
+      for (size_t q = 0; q < ev_used; ++q) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[q].a, ev[q].b));
+        tacc[2 * ev[q].kind] += ms;
+        tacc[2 * ev[q].kind + 1] += 1;
+      }
+      ev_used = 0;
+      timing_on = false;
+    } else if (enable) {
+      for (double& t : tacc) t = 0;
+      ev_used = 0;
+      timing_on = true;
+    }
+    if (out) std::memcpy(out, tacc, sizeof(tacc));
+    return CGFDM_OK;
+  }
+
+  // CFL inputs of the grid (calc_CFL.cpp:255-349): ranges = [hmin, hmax,
+  // vsmin, vsmax, vpmin, vpmax, rhomin, rhomax]
+  int grid_ranges(const float* dcoord, const float* dmed, int nx, int ny, int nz, int NX, int NY, long long N) {
+    const int nb = 1024;
+    double* part = nullptr;
+    int rc;
+    if ((rc = dalloc(&part, 2 * nb * 4))) return rc;
+    cg::launch_hrange(dcoord, part, nb, nx, ny, nz, NX, NY, N, stream);
+    for (int c = 0; c < 3; ++c) cg::launch_frange(dmed + c * N, N, part + 2 * nb * (c + 1), nb, stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(2 * nb * 4);
+    HIPCHK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int c = 0; c < 4; ++c) {
+      double lo = 1e300, hi = -1e300;
+      for (int b = 0; b < nb; ++b) {
+        lo = std::min(lo, h[2 * nb * c + 2 * b]);
+        hi = std::max(hi, h[2 * nb * c + 2 * b + 1]);
+      }
+      ranges[2 * c] = lo;
+      ranges[2 * c + 1] = hi;
+    }
+    dfree(part);
+    return CGFDM_OK;
+  }
+};
+
+template <class T>
+float to_float(T v) { return (float)v; }
+template <>
+float to_float<__half>(__half v) { return __half2float(v); }
+template <>
+float to_float<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
+
+template <class R, class TM>
+struct Solver : SolverBase {
+  cgfdm_desc d{};
+  int NX = 0, NY = 0, NZ = 0;
+  long long N = 0, S = 0;
+  double Z = 1.0;  // stress carried as T / Z
+  R *A = nullptr, *B = nullptr, *C = nullptr, *T = nullptr;
+  TM* coef = nullptr;
+  R* jac = nullptr;
+  R *rdzdx = nullptr, *rdzdy = nullptr;
+  R* pml_par = nullptr;  // alpha, beta, d for x (NX each), then y, then z
+  cg::PmlArgs<R> pml{};
+  float* pgv = nullptr;
+  R* dis = nullptr;
+  int ns = 0;
+  long long* st_cell = nullptr;
+  float* st_rec = nullptr;
+  int src_np = 0, src_nt = 0;
+  double src_dt = 0;
+  long long* src_cell = nullptr;
+  R* src_rate = nullptr;
   int init(const cgfdm_desc& dd, const float* coord, const float* medium) override {
     d = dd;
     NX = d.nx + 2 * cg::H;
@@ -148,25 +202,7 @@ struct Solver : SolverBase {
 
     // --- CFL ranges (calc_CFL.cpp:255-349)
     {
-      const int nb = 1024;
-      double* part = nullptr;
-      if ((rc = dalloc(&part, 2 * nb * 4))) return rc;
-      cg::launch_hrange(dcoord, part, nb, d.nx, d.ny, d.nz, NX, NY, N, stream);
-      for (int c = 0; c < 3; ++c) cg::launch_frange(dmed + c * N, N, part + 2 * nb * (c + 1), nb, stream);
-      HIPCHK(hipGetLastError());
-      std::vector<double> h(2 * nb * 4);
-      HIPCHK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipStreamSynchronize(stream));
-      for (int c = 0; c < 4; ++c) {
-        double lo = 1e300, hi = -1e300;
-        for (int b = 0; b < nb; ++b) {
-          lo = std::min(lo, h[2 * nb * c + 2 * b]);
-          hi = std::max(hi, h[2 * nb * c + 2 * b + 1]);
-        }
-        ranges[2 * c] = lo;
-        ranges[2 * c + 1] = hi;
-      }
-      dfree(part);
+      if ((rc = grid_ranges(dcoord, dmed, d.nx, d.ny, d.nz, NX, NY, N))) return rc;
       // ranges = [hmin, hmax, vsmin, vsmax, vpmin, vpmax, rhomin, rhomax]
       double dtmax = 1.34 * ranges[0] / ranges[5];
       if (d.dt > dtmax)
@@ -229,6 +265,7 @@ struct Solver : SolverBase {
       }
       if ((rc = dalloc(&pml_par, (long long)hp.size()))) return rc;
       HIPCHK(hipMemcpyAsync(pml_par, hp.data(), sizeof(R) * hp.size(), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));  // hp is a host temporary
       off = 0;
       for (int ax = 0; ax < 3; ++ax) {
         pml.alpha[ax] = pml_par + off;
@@ -330,17 +367,6 @@ struct Solver : SolverBase {
     return CGFDM_OK;
   }
 
-  size_t next_pair(int kind) {
-    if (ev_used >= ev.size()) {
-      EvPair e{};
-      hipEventCreate(&e.a);
-      hipEventCreate(&e.b);
-      ev.push_back(e);
-    }
-    ev[ev_used].kind = kind;
-    return ev_used++;
-  }
-
   int step(int n) override {
     const int kend = cg::H + d.nz - (d.free_surface ? 3 : 0);
     for (int s = 0; s < n; ++s) {
@@ -407,26 +433,6 @@ struct Solver : SolverBase {
       ++it;
     }
     HIPCHK(hipGetLastError());
-    return CGFDM_OK;
-  }
-
-  int timing(int enable, double* out) override {
-    if (timing_on && !enable) {
-      HIPCHK(hipStreamSynchronize(stream));
-      for (size_t q = 0; q < ev_used; ++q) {
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, ev[q].a, ev[q].b));
-        tacc[2 * ev[q].kind] += ms;
-        tacc[2 * ev[q].kind + 1] += 1;
-      }
-      ev_used = 0;
-      timing_on = false;
-    } else if (enable) {
-      for (double& t : tacc) t = 0;
-      ev_used = 0;
-      timing_on = true;
-    }
-    if (out) std::memcpy(out, tacc, sizeof(tacc));
     return CGFDM_OK;
   }
 
@@ -517,6 +523,252 @@ struct Solver : SolverBase {
   }
 };
 
+// ME-AWENO (SCFDM) solver: WENO5 alternative-flux finite differences on the
+// conservative state, TVD-RK3, characteristic free surface, exponential
+// sponge (propagate.cpp:186-321 under SCFDM). The state is kept in three
+// buffers that rotate through the stages, so the stage input, the state at
+// the step start and the stage output are always distinct arrays.
+template <class R>
+struct AwenoSolver : SolverBase {
+  cgfdm_desc d{};
+  int NX = 0, NY = 0, NZ = 0;
+  long long N = 0, S = 0;
+  R* buf[3] = {nullptr, nullptr, nullptr};
+  int cur = 0;
+  R* cjm = nullptr;
+  R* F = nullptr;
+  R alpha = R(0);
+  float* pgv = nullptr;
+  R* dis = nullptr;
+  int ns = 0;
+  long long* st_cell = nullptr;
+  float* st_rec = nullptr;
+  int src_np = 0, src_nt = 0;
+  double src_dt = 0;
+  long long* src_cell = nullptr;
+  R* src_rate = nullptr;
+  std::vector<double> jac_host;  // FP64 J for the source scaling
+
+  int init(const cgfdm_desc& dd, const float* coord, const float* medium) override {
+    d = dd;
+    NX = d.nx + 2 * cg::H;
+    NY = d.ny + 2 * cg::H;
+    NZ = d.nz + 2 * cg::H;
+    S = (long long)NX * NY;
+    N = S * NZ;
+    int rc;
+    float *dcoord = nullptr, *dmed = nullptr;
+    double* cov = nullptr;
+    if ((rc = dalloc(&dcoord, 3 * N))) return rc;
+    if ((rc = dalloc(&dmed, 3 * N))) return rc;
+    if ((rc = dalloc(&cov, 10 * N))) return rc;
+    HIPCHK(hipMemcpyAsync(dcoord, coord, sizeof(float) * 3 * N, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dmed, medium, sizeof(float) * 3 * N, hipMemcpyHostToDevice, stream));
+    if ((rc = grid_ranges(dcoord, dmed, d.nx, d.ny, d.nz, NX, NY, N))) return rc;
+    double dtmax = 1.34 * ranges[0] / ranges[5];
+    if (d.dt > dtmax)
+      return fail(CGFDM_ERR_CFL, "DT " + std::to_string(d.dt) + " exceeds the CFL limit " + std::to_string(dtmax));
+    alpha = R((float)ranges[5]);  // vp_max_for_SCFDM (calc_CFL.cpp:317-321)
+    cg::launch_cov_cmm(dcoord, cov, NX, NY, NZ, N, d.dh, stream);
+    if ((rc = dalloc(&cjm, (long long)aw::C_N * N))) return rc;
+    cg::launch_pack_cmm<R>(cov, dmed, cjm, NX, NY, NZ, N, stream);
+    HIPCHK(hipGetLastError());
+    jac_host.assign(N, 0.0);
+    HIPCHK(hipMemcpyAsync(jac_host.data(), cov + 9 * N, sizeof(double) * N, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    dfree(cov);
+    dfree(dmed);
+    dfree(dcoord);
+    for (int q = 0; q < 3; ++q)
+      if ((rc = dalloc(&buf[q], 9 * N))) return rc;
+    if ((rc = dalloc(&F, 27 * N))) return rc;
+    if ((rc = dalloc(&pgv, 2LL * d.nx * d.ny))) return rc;
+    if ((rc = dalloc(&dis, 3 * S))) return rc;
+    HIPCHK(hipStreamSynchronize(stream));
+    return CGFDM_OK;
+  }
+
+  int set_source(int np, int nt, double dt, const long long* cell, const float* rate) override {
+    if (np < 0 || (np > 0 && (nt < 2 || dt <= 0 || !cell || !rate))) return fail(CGFDM_ERR_SOURCE, "bad source table");
+    std::map<long long, std::vector<double>> acc;
+    for (int p = 0; p < np; ++p) {
+      long long c = cell[p];
+      int i = (int)(c % NX), j = (int)((c / NX) % NY), k = (int)(c / S);
+      if (c < 0 || c >= N || i < cg::H || i >= NX - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H ||
+          k >= NZ - cg::H)
+        return fail(CGFDM_ERR_SOURCE, "source cell outside the interior");
+      auto& v = acc[c];
+      if (v.empty()) v.assign((size_t)nt * 6, 0.0);
+      const float* r = rate + (size_t)p * nt * 6;
+      for (size_t q = 0; q < (size_t)nt * 6; ++q) v[q] += r[q];
+    }
+    int rc;
+    if (src_cell) dfree(src_cell);
+    if (src_rate) dfree(src_rate);
+    src_cell = nullptr;
+    src_rate = nullptr;
+    src_np = (int)acc.size();
+    src_nt = nt;
+    src_dt = dt;
+    if (src_np == 0) return CGFDM_OK;
+    std::vector<long long> hc;
+    std::vector<R> hr((size_t)src_np * nt * 6);
+    int Jp = 0;
+    for (auto& kv : acc) {
+      long long c = kv.first;
+      hc.push_back(c);
+      R mv;
+      HIPCHK(hipMemcpy(&mv, cjm + (long long)aw::C_MU * N + c, sizeof(R), hipMemcpyDeviceToHost));
+      double sc = (double)mv / (jac_host[c] * d.dh * d.dh * d.dh);  // mu/(J DH^3) (addMoment.cpp:27-76)
+      for (int t = 0; t < nt; ++t)
+        for (int q = 0; q < 6; ++q) hr[((size_t)t * src_np + Jp) * 6 + q] = R(kv.second[(size_t)t * 6 + q] * sc);
+      ++Jp;
+    }
+    if ((rc = dalloc(&src_cell, src_np))) return rc;
+    if ((rc = dalloc(&src_rate, (long long)hr.size()))) return rc;
+    HIPCHK(hipMemcpy(src_cell, hc.data(), sizeof(long long) * src_np, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(src_rate, hr.data(), sizeof(R) * hr.size(), hipMemcpyHostToDevice));
+    return CGFDM_OK;
+  }
+
+  int set_stations(int n, const int* ijk) override {
+    if (n < 0 || (n > 0 && !ijk)) return fail(CGFDM_ERR_ARG, "bad station list");
+    std::vector<long long> hc(n);
+    for (int s = 0; s < n; ++s) {
+      int i = ijk[3 * s], j = ijk[3 * s + 1], k = ijk[3 * s + 2];
+      if (i < cg::H || i >= NX - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H || k >= NZ - cg::H)
+        return fail(CGFDM_ERR_ARG, "station outside the interior");
+      hc[s] = (long long)i + (long long)NX * j + (long long)k * S;
+    }
+    int rc;
+    if (st_cell) dfree(st_cell);
+    if (st_rec) dfree(st_rec);
+    st_cell = nullptr;
+    st_rec = nullptr;
+    ns = n;
+    if (n == 0) return CGFDM_OK;
+    if ((rc = dalloc(&st_cell, n))) return rc;
+    if ((rc = dalloc(&st_rec, (long long)n * d.nt * 9))) return rc;
+    HIPCHK(hipMemcpy(st_cell, hc.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    return CGFDM_OK;
+  }
+
+  int step(int n) override {
+    for (int s = 0; s < n; ++s) {
+      size_t sp = 0;
+      if (timing_on) {
+        sp = next_pair(2);
+        hipEventRecord(ev[sp].a, stream);
+      }
+      R* W = buf[cur];
+      if (src_np > 0) {
+        float DTf = (float)d.dt;
+        float tmpT = ((float)it + 0.0f) * DTf;
+        int srcIt = int(tmpT / (float)src_dt);
+        if (srcIt + 1 < src_nt) {
+          float t1 = float(srcIt) * (float)src_dt;
+          float tw = (tmpT - t1) / (float)src_dt;
+          aw::launch_source<R>(W, cjm, N, src_np, src_cell, src_rate, srcIt, R(tw), R(d.dt), stream);
+        }
+      }
+      const int o1 = (cur + 1) % 3, o2 = (cur + 2) % 3;
+      R* ins[3] = {buf[cur], buf[o1], buf[o2]};
+      R* outs[3] = {buf[o1], buf[o2], buf[o1]};
+      for (int irk = 0; irk < 3; ++irk) {
+        aw::StageArgs<R> a{};
+        a.NX = NX;
+        a.NY = NY;
+        a.NZ = NZ;
+        a.N = N;
+        a.S = S;
+        a.stage = irk;
+        a.DT = R(d.dt);
+        a.rDH = R(1.0 / d.dh);
+        a.alpha = alpha;
+        a.cjm = cjm;
+        a.win = ins[irk];
+        a.wm = buf[cur];
+        a.wout = outs[irk];
+        a.F = F;
+        size_t fp = 0, up = 0;
+        if (timing_on) {
+          fp = next_pair(0);
+          hipEventRecord(ev[fp].a, stream);
+        }
+        aw::launch_faces<R>(a, stream);
+        if (timing_on) {
+          hipEventRecord(ev[fp].b, stream);
+          up = next_pair(1);
+          hipEventRecord(ev[up].a, stream);
+        }
+        aw::launch_update<R>(a, d.nx, d.ny, stream);
+        if (timing_on) hipEventRecord(ev[up].b, stream);
+      }
+      cur = o1;
+      W = buf[cur];
+      if (d.free_surface) aw::launch_char_surface<R>(W, cjm, d.nx, d.ny, NX, NZ, N, S, stream);
+      aw::launch_sponge<R>(W, d.nx, d.ny, d.nz, NX, N, S, d.free_surface ? 1 : 0, stream);
+      if (d.free_surface) aw::launch_surface_obs<R>(W, cjm, pgv, dis, d.nx, d.ny, NX, NZ, N, S, R(d.dt), stream);
+      aw::launch_record<R>(W, cjm, N, ns, st_cell, st_rec, d.nt, it, stream);
+      if (timing_on) hipEventRecord(ev[sp].b, stream);
+      ++it;
+    }
+    HIPCHK(hipGetLastError());
+    return CGFDM_OK;
+  }
+
+  int get_stations(float* out) override {
+    if (ns == 0) return CGFDM_OK;
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(out, st_rec, sizeof(float) * (size_t)ns * d.nt * 9, hipMemcpyDeviceToHost));
+    return CGFDM_OK;
+  }
+
+  int get_pgv(float* out) override {
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(out, pgv, sizeof(float) * 2 * (size_t)d.nx * d.ny, hipMemcpyDeviceToHost));
+    return CGFDM_OK;
+  }
+
+  int get_dis(float* out) override {
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<R> h(3 * (size_t)S);
+    HIPCHK(hipMemcpy(h.data(), dis, sizeof(R) * h.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < 3; ++c)
+      for (int j = 0; j < d.ny; ++j)
+        for (int i = 0; i < d.nx; ++i)
+          out[((size_t)c * d.ny + j) * d.nx + i] = (float)h[(size_t)c * S + (size_t)(j + cg::H) * NX + (i + cg::H)];
+    return CGFDM_OK;
+  }
+
+  int get_field(int comp, double* out) override {
+    if (comp < 0 || comp > 8) return fail(CGFDM_ERR_ARG, "component out of range");
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<R> h(N);
+    HIPCHK(hipMemcpy(h.data(), buf[cur] + (size_t)comp * N, sizeof(R) * N, hipMemcpyDeviceToHost));
+    for (long long p = 0; p < N; ++p) out[p] = (double)h[p];
+    return CGFDM_OK;
+  }
+
+  int set_field(int comp, const double* in) override {
+    if (comp < 0 || comp > 8) return fail(CGFDM_ERR_ARG, "component out of range");
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<R> h(N);
+    for (long long p = 0; p < N; ++p) h[p] = R(in[p]);
+    HIPCHK(hipMemcpy(buf[cur] + (size_t)comp * N, h.data(), sizeof(R) * N, hipMemcpyHostToDevice));
+    return CGFDM_OK;
+  }
+
+  int get_setup(int what, double* out) override {
+    if (what < 0 || what >= aw::C_N) return fail(CGFDM_ERR_ARG, "unknown setup product");
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<R> h(N);
+    HIPCHK(hipMemcpy(h.data(), cjm + (size_t)what * N, sizeof(R) * N, hipMemcpyDeviceToHost));
+    for (long long p = 0; p < N; ++p) out[p] = (double)h[p];
+    return CGFDM_OK;
+  }
+};
+
 }  // namespace
 
 struct cgfdm_solver {
@@ -536,9 +788,14 @@ int cgfdm_create(const cgfdm_desc* desc, const float* coord, const float* medium
   if (d.npml < 0) return fail(CGFDM_ERR_ARG, "negative npml");
   if (d.npml > 0 && (d.npml >= d.nx || d.npml >= d.ny || d.npml >= d.nz))
     return fail(CGFDM_ERR_PML, "nPML must be smaller than nx, ny and nz");
-  if (d.scheme != CGFDM_SCHEME_CGFDM) return fail(CGFDM_ERR_ARG, "scheme not available in this build");
+  if (d.scheme != CGFDM_SCHEME_CGFDM && d.scheme != CGFDM_SCHEME_AWENO) return fail(CGFDM_ERR_ARG, "unknown scheme");
   std::unique_ptr<SolverBase> s;
-  if (d.precision == 8) {
+  if (d.scheme == CGFDM_SCHEME_AWENO) {
+    if (d.coef_storage != 0) return fail(CGFDM_ERR_ARG, "16-bit coefficient storage is a CGFDM option");
+    if (d.precision == 8) s.reset(new AwenoSolver<double>());
+    else if (d.precision == 4) s.reset(new AwenoSolver<float>());
+    else return fail(CGFDM_ERR_ARG, "precision must be 4 or 8");
+  } else if (d.precision == 8) {
     if (d.coef_storage != 0) return fail(CGFDM_ERR_ARG, "16-bit coefficient storage needs the FP32 wavefield");
     s.reset(new Solver<double, double>());
   } else if (d.precision == 4) {

@@ -12,6 +12,9 @@ void launch_pack(const double* met, const float* med, TM* coef, R* jac, long lon
 template <class R>
 void launch_rdz(const double* met, const float* med, R* rdzdx, R* rdzdy, int nx, int ny, int nz, int NX, int NY,
                 long long N, hipStream_t st);
+void launch_cov_cmm(const float* coord, double* cov, int NX, int NY, int NZ, long long N, double dh, hipStream_t st);
+template <class R>
+void launch_pack_cmm(const double* cov, const float* med, R* cjm, int NX, int NY, int NZ, long long N, hipStream_t st);
 void launch_hrange(const float* coord, double* part, int nblocks, int nx, int ny, int nz, int NX, int NY, long long N,
                    hipStream_t st);
 void launch_frange(const float* a, long long n, double* part, int nblocks, hipStream_t st);

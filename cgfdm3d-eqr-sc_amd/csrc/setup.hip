@@ -155,6 +155,55 @@ __global__ void k_rdz(const double* __restrict__ met, const float* __restrict__ 
     }
 }
 
+// SCFDM metrics (contravariant.cpp:81-177): covariant basis by second-order
+// central differences and J = det on [1, n-1) of every padded axis. Output:
+// 10 FP64 planes (x_xi, y_xi, z_xi, x_et, y_et, z_et, x_zt, y_zt, z_zt, J).
+__global__ void k_cov_cmm(const float* __restrict__ coord, double* __restrict__ cov, int NX, int NY, int NZ,
+                          long long N, double rdh) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y;
+  int k = 1 + blockIdx.z;
+  if (i >= NX - 1) return;
+  const long long S = (long long)NX * NY;
+  const long long p = (long long)i + (long long)NX * j + (long long)k * S;
+  const long long st[3] = {1, NX, S};
+  double d[9];
+  for (int dir = 0; dir < 3; ++dir)
+    for (int a = 0; a < 3; ++a) {
+      const float* c = coord + a * N;
+      d[3 * dir + a] = 0.5 * ((double)c[p + st[dir]] - (double)c[p - st[dir]]) * rdh;
+    }
+  for (int q = 0; q < 9; ++q) cov[q * N + p] = d[q];
+  cov[9 * N + p] = d[0] * d[4] * d[8] + d[1] * d[5] * d[6] + d[2] * d[3] * d[7] - d[2] * d[4] * d[6] -
+                   d[1] * d[3] * d[8] - d[0] * d[7] * d[5];
+}
+
+// SCFDM coefficient planes (CJM.cpp:107-147): cofactors J*xi_x .. J*zt_z on
+// [2, n-2), J and the tangents x_xi, x_et on [1, n-1), zero elsewhere; medium
+// (mu, lambda, 1/rho) everywhere.
+template <class R>
+__global__ void k_pack_cmm(const double* __restrict__ cov, const float* __restrict__ med, R* __restrict__ cjm, int NX,
+                           int NY, int NZ, long long N) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const long long S = (long long)NX * NY;
+  const int i = (int)(p % NX), j = (int)((p / NX) % NY), k = (int)(p / S);
+  const bool in1 = i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1 && k >= 1 && k < NZ - 1;
+  const bool in2 = i >= 2 && i < NX - 2 && j >= 2 && j < NY - 2 && k >= 2 && k < NZ - 2;
+  double d[9];
+  for (int q = 0; q < 9; ++q) d[q] = in1 ? cov[q * N + p] : 0.0;
+  const double cof[9] = {d[4] * d[8] - d[5] * d[7], d[5] * d[6] - d[3] * d[8], d[3] * d[7] - d[4] * d[6],
+                         d[7] * d[2] - d[8] * d[1], d[8] * d[0] - d[6] * d[2], d[6] * d[1] - d[7] * d[0],
+                         d[1] * d[5] - d[2] * d[4], d[2] * d[3] - d[0] * d[5], d[0] * d[4] - d[1] * d[3]};
+  for (int q = 0; q < 9; ++q) cjm[q * N + p] = in2 ? (R)cof[q] : R(0);
+  cjm[9 * N + p] = in1 ? (R)cov[9 * N + p] : R(0);
+  for (int q = 0; q < 6; ++q) cjm[(13 + q) * N + p] = (R)d[q];
+  const double vs = med[p], vp = med[N + p], rho = med[2 * N + p];
+  cjm[10 * N + p] = (R)(rho * vs * vs);
+  cjm[11 * N + p] = (R)(rho * (vp * vp - 2.0 * vs * vs));
+  cjm[12 * N + p] = (R)(1.0 / rho);
+}
+
 // Minimum / maximum point-to-plane distance over the interior (calc_CFL.cpp:56-163),
 // one partial per block.
 __global__ void k_hrange(const float* __restrict__ coord, double* part, int nx, int ny, int nz, int NX, int NY,
@@ -253,6 +302,16 @@ void launch_rdz(const double* met, const float* med, R* rdzdx, R* rdzdy, int nx,
   k_rdz<R><<<g, 64, 0, st>>>(met, med, rdzdx, rdzdy, nx, ny, nz, NX, NY, N);
 }
 
+void launch_cov_cmm(const float* coord, double* cov, int NX, int NY, int NZ, long long N, double dh, hipStream_t st) {
+  dim3 g((NX - 2 + 63) / 64, NY - 2, NZ - 2);
+  k_cov_cmm<<<g, 64, 0, st>>>(coord, cov, NX, NY, NZ, N, 1.0 / dh);
+}
+
+template <class R>
+void launch_pack_cmm(const double* cov, const float* med, R* cjm, int NX, int NY, int NZ, long long N, hipStream_t st) {
+  k_pack_cmm<R><<<(unsigned)((N + 255) / 256), 256, 0, st>>>(cov, med, cjm, NX, NY, NZ, N);
+}
+
 void launch_hrange(const float* coord, double* part, int nblocks, int nx, int ny, int nz, int NX, int NY, long long N,
                    hipStream_t st) {
   k_hrange<<<nblocks, 256, 0, st>>>(coord, part, nx, ny, nz, NX, NY, N);
@@ -272,5 +331,8 @@ template void launch_rdz<float>(const double*, const float*, float*, float*, int
                                 hipStream_t);
 template void launch_rdz<double>(const double*, const float*, double*, double*, int, int, int, int, int, long long,
                                  hipStream_t);
+
+template void launch_pack_cmm<float>(const double*, const float*, float*, int, int, int, long long, hipStream_t);
+template void launch_pack_cmm<double>(const double*, const float*, double*, int, int, int, long long, hipStream_t);
 
 }  // namespace cg

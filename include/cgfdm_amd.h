@@ -39,17 +39,21 @@ typedef struct cgfdm_solver cgfdm_solver;
 
 typedef struct cgfdm_desc {
   int nx, ny, nz;     /* interior cells of the domain (grid.nx/ny/nz, init_grid.cpp:37-45 with P = 1) */
-  int npml;           /* params.nPML (getParams.cpp:137-138); 0 = no absorbing layer */
+  int npml;           /* params.nPML (getParams.cpp:137-138); 0 = no absorbing layer. CGFDM only:
+                         the AWENO scheme absorbs with the 20-layer exponential sponge of the
+                         SCFDM build (exp_decay_absorb.cpp:108-145) and ignores npml */
   int free_surface;   /* 1 = FREE_SURFACE build (traction image on the top face) */
   int scheme;         /* CGFDM_SCHEME_* */
   int precision;      /* 4 = FP32 wavefield (reference FLOAT), 8 = FP64 wavefield */
   int coef_storage;   /* 0 = wavefield precision, 1 = fp16, 2 = bf16 storage of metric and medium
-                         planes, with stress carried as T / (rho_ref Vp_ref) (macro.h:75-80 Cs) */
+                         planes, with stress carried as T / (rho_ref Vp_ref) (macro.h:75-80 Cs);
+                         CGFDM with the FP32 wavefield only */
   int nt;             /* steps the station buffers hold, NT = TMAX / DT (propagate.cpp:88) */
   double dh, dt;      /* params.DH, params.DT */
 } cgfdm_desc;
 
-/* Setup: FP64 metrics/Jacobian (contravariant.cpp:182-285), metric mirror and
+/* Setup: FP64 metrics/Jacobian (contravariant.cpp:182-285; SCFDM: central-difference
+ * covariant basis and cofactors, contravariant.cpp:81-177), metric mirror and
  * traction matrices (contravariant.cpp:321-519), (Vs,Vp,rho) -> (mu,lambda,1/rho)
  * and CJM packing (CJM.cpp:53-181), CFL check (calc_CFL.cpp:255-349), PML
  * profiles (init_pml_para.cpp:97-285) and wave/aux allocation (wave_deriv.cpp:31-69,
@@ -71,7 +75,10 @@ int cgfdm_set_stations(cgfdm_solver* s, int ns, const int* ijk);
 
 /* Advance nsteps full time steps: source injection at t = it*DT, RK stages with
  * derivative, free surface, PML and update, then station/PGV/displacement
- * recording (propagate.cpp:186-321). Asynchronous on the solver stream. */
+ * recording (propagate.cpp:186-321). AWENO: source as strain rate, three TVD-RK3
+ * stages of WENO5/Lax-Friedrichs alternative-flux differences
+ * (alternative_flux_FD.cpp:1437-1569, wave_rk_tvd.cpp), characteristic free
+ * surface, exponential sponge, observers. Asynchronous on the solver stream. */
 int cgfdm_step(cgfdm_solver* s, int nsteps);
 
 /* Use an existing HIP stream (hipStream_t passed as void*); NULL = default stream. */
@@ -88,21 +95,26 @@ int cgfdm_get_pgv(cgfdm_solver* s, float* out);
 /* Surface displacement float[3][ny][nx] (FreeSurfU*, solve_displacement.cpp:28-74). */
 int cgfdm_get_surface_displacement(cgfdm_solver* s, float* out);
 /* One wavefield component of the current state, double[NZ][NY][NX] in physical
- * units; source of the slice output (data_io.cpp:485-587). */
+ * units; source of the slice output (data_io.cpp:485-587). CGFDM: (Vx,Vy,Vz,Txx,
+ * Tyy,Tzz,Txy,Txz,Tyz); AWENO: the conservative state (rho vx, rho vy, rho vz,
+ * exx, eyy, ezz, gyz, gxz, gxy) (station records and PGV are physical for both). */
 int cgfdm_get_field(cgfdm_solver* s, int comp, double* out);
 /* Overwrite one wavefield component (initial conditions; no reference counterpart). */
 int cgfdm_set_field(cgfdm_solver* s, int comp, const double* in);
 
 /* Setup products for inspection: what = 0..8 metrics, 9 J, 10 mu, 11 lambda, 12 1/rho
  * (double[NZ][NY][NX]); 20..28 rDZ_DX, 30..38 rDZ_DY (double[NY][NX]); 40 + 3*axis +
- * {0 alpha, 1 1/beta, 2 d/beta} (double[NX|NY|NZ]). (CJM.cpp:107-147, init_pml_para.cpp) */
+ * {0 alpha, 1 1/beta, 2 d/beta} (double[NX|NY|NZ]). (CJM.cpp:107-147, init_pml_para.cpp)
+ * AWENO: what = 0..18, the SCFDM CJM planes (0..8 J*xi_x..J*zt_z, 9 J, 10..12 medium,
+ * 13..18 x_xi, y_xi, z_xi, x_et, y_et, z_et), double[NZ][NY][NX]. */
 int cgfdm_get_setup(cgfdm_solver* s, int what, double* out);
 /* [Hmin, Hmax, Vs_min, Vs_max, Vp_min, Vp_max, rho_min, rho_max] (calc_CFL.cpp:316-340). */
 int cgfdm_get_ranges(cgfdm_solver* s, double* out8);
 
 /* Per-kernel timing with HIP events on the solver stream. enable = 1 starts
  * accumulation (and clears it), 0 stops. out[0..5] = {interior ms, interior
- * launches, surface ms, surface launches, step ms, steps}; out may be NULL. */
+ * launches, surface ms, surface launches, step ms, steps}; out may be NULL.
+ * AWENO: {face-flux ms, launches, update ms, launches, step ms, steps}. */
 int cgfdm_timing(cgfdm_solver* s, int enable, double* out);
 
 /* CFL inputs without a solver: [Hmin, Hmax, Vs_min, Vs_max, Vp_min, Vp_max, rho_min,
