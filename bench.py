@@ -180,7 +180,7 @@ def main():
         "l2_err_fp32_vs_fp64": l2,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "cg_interior", "launch_ms": launch_ms,
+                     "kernel": "cg_cell", "launch_ms": launch_ms,
                      "alg_bytes_per_cell": stage_bytes(args.coef), "cells_per_launch": cells_launch},
         "cpu_baseline": cpu,
     }

@@ -298,14 +298,15 @@ def config2(nt=3000, nx=512, ny=512, nz=256, dh=100.0, npml=20, seed=1002, nhill
 
 def config3(nt=4000, nx=512, ny=512, nz=256, dh=100.0, seed=1002, nhills=8, hmin_fn=None):
     """ME-AWENO: the 512x512x256 terrain grid of config2 with a sharp
-    low-velocity basin (Vs 400, Vp 1200, rho 1900 inside an ellipsoid of
-    20 x 10 x 2 km extent under the centre of the surface; Vs 3464, Vp 6000,
-    rho 2700 outside), point moment source at 5 km depth, 400 surface stations.
+    low-velocity basin (Vs 400, Vp 1200, rho 1900 inside the ellipsoid of
+    semi-axes 10 x 5 x 1 km centred on the surface at the domain centre, i.e. a
+    20 x 10 x 2 km ellipsoid cut by the surface; Vs 3464, Vp 6000, rho 2700
+    outside), point moment source at 5 km depth, 400 surface stations.
     Absorption is the scheme's 20-layer exponential sponge."""
     depth = nz * dh
     terrain = gaussian_hills(nx, ny, dh, 20, nhills, seed)
     coord = terrain_coord(nx, ny, nz, dh, terrain, depth)
-    med = basin_medium(coord, terrain, ((nx // 2) * dh, (ny // 2) * dh), (10000.0, 5000.0, 2000.0))
+    med = basin_medium(coord, terrain, ((nx // 2) * dh, (ny // 2) * dh), (10000.0, 5000.0, 1000.0))
     hmin = (hmin_fn or (lambda c, md: hmin_point_plane(c)))(coord, med)
     dt = 0.5 * hmin / float(med[1].max())
     m = Model(nx, ny, nz, dh, dt, nt, 0, coord, med, True, name="config3", terrain=terrain)
