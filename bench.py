@@ -10,6 +10,9 @@ Inputs are resident in HBM before the timed region starts.
 
 Prints one JSON line (see the keys below). Per-kernel times come from HIP events
 recorded on the solver's stream inside the timed region.
+
+`--scheme aweno` measures the ME-AWENO step instead (the 512 x 512 x 256 terrain
+grid with a sharp low-velocity basin, TVD-RK3, sponge) and prints its own line.
 """
 import argparse
 import json
@@ -53,11 +56,19 @@ def traffic_from_profiles(name):
         return None
 
 
-def cpu_baseline(model, steps, threads):
+# ME-AWENO algorithmic bytes per cell and TVD stage (FP32): face kernel reads W
+# (36 B) and 12 coefficients (48 B) and writes 27 face fluxes (108 B); the update
+# reads the 27 fluxes, J, the stage input and the step-start state and writes
+# the stage output (108 + 4 + 3 * 36 B).
+AW_FACE_BYTES = 36 + 48 + 108
+AW_UPDATE_BYTES = 108 + 4 + 3 * 36
+
+
+def cpu_baseline(model, steps, threads, scheme="cgfdm"):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    log(f"cpu baseline: oracle FP32 on {threads} threads, {steps} steps of the same grid")
-    o = oracle.Oracle(model, prec=4, threads=threads)
+    log(f"cpu baseline: oracle FP32 ({scheme}) on {threads} threads, {steps} steps of the same grid")
+    o = (oracle.AwenoOracle if scheme == "aweno" else oracle.Oracle)(model, prec=4, threads=threads)
     o.step(1)  # first touch
     t0 = time.perf_counter()
     o.step(steps)
@@ -78,6 +89,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--scheme", choices=["cgfdm", "aweno"], default="cgfdm")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -91,6 +103,8 @@ def main():
     from cgfdm_amd import model as M
     from cgfdm_amd import solver as S
 
+    if args.scheme == "aweno":
+        return bench_aweno(args)
     t0 = time.perf_counter()
     hmin_gpu = lambda c, md: S.grid_ranges(args.nx, args.ny, args.nz, c, md)["hmin"]
     m = M.config2(nt=args.warmup + args.steps + 1, nx=args.nx, ny=args.ny, nz=args.nz, hmin_fn=hmin_gpu)
@@ -182,6 +196,79 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "cg_cell", "launch_ms": launch_ms,
                      "alg_bytes_per_cell": stage_bytes(args.coef), "cells_per_launch": cells_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_aweno(args):
+    import numpy as np
+    import torch
+    from cgfdm_amd import model as M
+    from cgfdm_amd import solver as S
+
+    t0 = time.perf_counter()
+    hmin_gpu = lambda c, md: S.grid_ranges(args.nx, args.ny, args.nz, c, md)["hmin"]
+    m = M.config3(nt=args.warmup + args.steps + 1, nx=args.nx, ny=args.ny, nz=args.nz, hmin_fn=hmin_gpu)
+    log(f"model {m.nx}x{m.ny}x{m.nz} (basin) built in {time.perf_counter() - t0:.1f}s, dt={m.dt:.4e}")
+    stream = torch.cuda.Stream(device=torch.device("cuda:0"))
+    g = S.Solver(m, precision=4, scheme=S.SCHEME_AWENO)
+    g.set_stream(stream)
+    log(f"solver ready: {g.device_bytes() / 2**30:.1f} GiB on device")
+    g.step(args.warmup)
+    g.synchronize()
+    torch.cuda.synchronize()
+    g.timing(True)
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tm = g.timing(False)
+    log(f"timed {args.steps} steps: wall {wall:.3f}s; kernels {tm}")
+    ms_step = wall * 1e3 / args.steps
+    gcells = m.cells * args.steps / wall / 1e9
+    NX, NY, NZ = m.shape_padded
+    face_cells = (NX - 5) * (NY - 5) * (NZ - 5)
+    launch_ms = tm["interior_ms"] / max(tm["interior_n"], 1)
+    upd_ms = tm["surface_ms"] / max(tm["surface_n"], 1)
+    achieved = AW_FACE_BYTES * face_cells / (launch_ms * 1e-3) / 1e9
+
+    l2 = None
+    if not args.no_accuracy:
+        m1 = M.small_basin_config(nx=96, ny=96, nz=48, nt=200)
+        a = S.Solver(m1, precision=4, scheme=S.SCHEME_AWENO)
+        a.step(200)
+        b = S.Solver(m1, precision=8, scheme=S.SCHEME_AWENO)
+        b.step(200)
+        sa, sb = a.stations(), b.stations()
+        l2 = float(np.linalg.norm(sa[:, :, :3] - sb[:, :, :3]) / np.linalg.norm(sb[:, :, :3]))
+        a.close()
+        b.close()
+        log(f"accuracy: basin 96x96x48, 200 steps, FP32 vs FP64 station velocity rel L2 = {l2:.3e}")
+    g.close()
+    torch.cuda.empty_cache()
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        v, secs = cpu_baseline(m, args.cpu_steps, threads, scheme="aweno")
+        cpu = {"value": v, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"CPU oracle (C++ restatement, FP32, OpenMP) of the ME-AWENO step on the same "
+                         f"{m.nx}x{m.ny}x{m.nz} grid, {args.cpu_steps} full steps ({secs:.1f}s)"}
+    out = {
+        "metric": METRIC, "value": gcells, "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"ME-AWENO {m.nx}x{m.ny}x{m.nz} h=100m, 8 Gaussian hills, sharp basin "
+                               f"(Vs 400 in Vs 3464), sponge 20, free surface, point source, "
+                               f"{len(m.stations)} stations + PGV, TVD-RK3", "cells": m.cells, "dt": m.dt},
+        "l2_err_fp32_vs_fp64": l2,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "aw_faces",
+                     "launch_ms": launch_ms, "update_launch_ms": upd_ms,
+                     "alg_bytes_per_cell": AW_FACE_BYTES, "cells_per_launch": face_cells,
+                     "note": "aw_faces is VALU-bound (18 WENO5 interpolations per face); the HBM fraction is "
+                             "reported for the algorithmic bytes"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

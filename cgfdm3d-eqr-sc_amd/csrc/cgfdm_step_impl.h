@@ -94,14 +94,13 @@ __device__ __forceinline__ void pml_face(const FaceBuf<R>& F, long long pos, con
 }
 
 // ---------------------------------------------------------------------------
-// interior kernel: one thread per cell below the surface layers
+// interior cell update: 27 derivatives, PML faces, RK4 stage. `zd(v)` returns
+// the unscaled zeta difference of field v and `wc(v)` the centre value, so the
+// same body serves the thread-per-cell and the z-marching kernels.
 // ---------------------------------------------------------------------------
-template <class R, class TM, int FB1, int FB2, int FB3>
-__global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restrict__ coef) {
-  const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int k = a.kbeg + blockIdx.z;
-  if (i >= H + a.nx || j >= H + a.ny || k >= a.kend) return;
+template <class R, class TM, int FB1, int FB2, int FB3, class ZD, class WC>
+__device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __restrict__ coef, int i, int j, int k,
+                                          ZD zd, WC wc) {
   const long long N = a.N, S = a.S;
   const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
   const R* W = a.win;
@@ -151,7 +150,7 @@ __global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restr
   }
   // z: derivatives along zeta (wave_deriv.cpp:234-242)
 #pragma unroll
-  for (int v = 0; v < 9; ++v) D[v] = mac<FB3>(W + (long long)v * N + p, S);
+  for (int v = 0; v < 9; ++v) D[v] = zd(v);
   assemble(D, cf(M_ZTX), cf(M_ZTY), cf(M_ZTZ), mu, lam, bu, Q);
   const R bz = P.beta[2][k];
 #pragma unroll
@@ -173,7 +172,7 @@ __global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restr
   for (int v = 0; v < 9; ++v) {
     const long long q = (long long)v * N + p;
     if (a.stage == 0) {
-      const R m = W[q];
+      const R m = wc(v);
       a.wt[q] = m + R(RK_B1) * h[v];
       a.wout[q] = m + R(RK_A2) * h[v];
     } else if (a.stage == 1) {
@@ -186,6 +185,23 @@ __global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restr
       a.wout[q] = a.wt[q] + R(RK_B4) * h[v];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// interior kernel: one thread per cell below the surface layers
+// ---------------------------------------------------------------------------
+template <class R, class TM, int FB1, int FB2, int FB3>
+__global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restrict__ coef) {
+  const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int k = a.kbeg + blockIdx.z;
+  if (i >= H + a.nx || j >= H + a.ny || k >= a.kend) return;
+  const long long N = a.N, S = a.S;
+  const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
+  const R* W = a.win;
+  cell_body<R, TM, FB1, FB2, FB3>(
+      a, coef, i, j, k, [&](int v) -> R { return mac<FB3>(W + (long long)v * N + p, S); },
+      [&](int v) -> R { return W[(long long)v * N + p]; });
 }
 
 // ---------------------------------------------------------------------------
