@@ -10,7 +10,7 @@ sys.path.insert(0, ROOT)
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
     config.addinivalue_line("markers", "slow: long CPU test")
 
 
