@@ -97,6 +97,20 @@ def test_fp32_observers_match_fp64_oracle(oracle_mod, solver_mod):
     assert rel_l2(g.surface_displacement(), o64.surface_dis()[:, H:-H, H:-H]) < 1e-4
 
 
+@pytest.mark.parametrize("coef,tol", [(1, 5e-3), (2, 4e-2)])
+def test_16bit_coefficients_vs_fp64_oracle(coef, tol, oracle_mod, solver_mod):
+    """FP32 wavefield with FP16 (coef=1) or BF16 (coef=2) storage of the metric and
+    medium planes, stress carried in units of a reference impedance: station
+    seismograms against the FP64 oracle within the coefficient rounding
+    (2^-11, 2^-8 relative) times a small amplification."""
+    m = make("curvilinear_pml_fs", nt=40)
+    g, o64 = run_pair(m, oracle_mod, solver_mod, 4, 40, coef_storage=coef)
+    sg, s64 = g.stations(), o64.stations()
+    for c in range(9):
+        assert rel_l2(sg[:, :, c], s64[:, :, c]) < tol, (c, rel_l2(sg[:, :, c], s64[:, :, c]))
+    assert rel_l2(g.pgv(), o64.pgv()) < tol
+
+
 def test_error_codes(solver_mod):
     m = make("no_pml_flat")
     m.dt = m.dt * 10.0
