@@ -74,6 +74,9 @@ struct Oracle {
   double src_dt = 0;
   std::vector<long long> src_idx;
   std::vector<T> src_rate;
+  std::vector<double> src_raw;      // merged, unscaled, time-major [t][p][6]
+  std::vector<double> src_vol;      // 1 / (J DH^3) per merged point
+  std::vector<int> src_of_point;    // merged index of every input point
   // observers
   int ns = 0, NTrec = 0;
   std::vector<int> st_ijk;
@@ -165,8 +168,8 @@ struct Oracle {
     rdzdy.assign(9 * (size_t)NX * NY, T(0));
     if (!fs) return;
     const float* VS = medium;
-    const float* VP = medium + N;
-    const float* RHO = medium + 2 * N;
+    const float* VP = medium ? medium + N : nullptr;
+    const float* RHO = medium ? medium + 2 * N : nullptr;
     int k = NZ - H - 1;
     const long long S = (long long)NX * NY;
     for (int j = H; j < NY - H; ++j)
@@ -176,8 +179,15 @@ struct Oracle {
         double xx = m[0 * N + p], xy = m[1 * N + p], xz = m[2 * N + p];
         double ex = m[3 * N + p], ey = m[4 * N + p], ez = m[5 * N + p];
         double zx = m[6 * N + p], zy = m[7 * N + p], zz = m[8 * N + p];
-        double vs = VS[p], vp = VP[p], rho = RHO[p];
-        double mu = rho * vs * vs, lam = rho * (vp * vp - 2.0 * vs * vs);
+        double mu, lam;
+        if (medium) {
+          double vs = VS[p], vp = VP[p], rho = RHO[p];
+          mu = rho * vs * vs;
+          lam = rho * (vp * vp - 2.0 * vs * vs);
+        } else {
+          mu = (double)cjm[C_MU * N + p];
+          lam = (double)cjm[C_LAM * N + p];
+        }
         double chi = 2.0 * lam + mu, lm = lam + mu;
         double DZ[9], DX[9], DY[9], R[9];
         DZ[0] = chi * zx * zx + mu * (zy * zy + zz * zz);
@@ -326,19 +336,45 @@ struct Oracle {
     src_nt = nt;
     src_dt = dt;
     src_idx.clear();
-    src_rate.assign((size_t)src_np * nt * 6, T(0));
+    src_raw.assign((size_t)src_np * nt * 6, 0.0);
+    src_vol.assign(src_np, 0.0);
+    std::map<long long, int> merged;
     int J = 0;
     for (auto& kv : acc) {
       long long c = kv.first;
+      merged[c] = J;
       src_idx.push_back(c);
-      double jac = metric_d[9 * N + c];
-      double mu = (double)cjm[C_MU * N + c];
-      double V = 1.0 / (jac * dh * dh * dh);
+      src_vol[J] = 1.0 / (metric_d[9 * N + c] * dh * dh * dh);
       for (int t = 0; t < nt; ++t)
-        for (int q = 0; q < 6; ++q)
-          src_rate[((size_t)t * src_np + J) * 6 + q] = T(kv.second[(size_t)t * 6 + q] * mu * V);
+        for (int q = 0; q < 6; ++q) src_raw[((size_t)t * src_np + J) * 6 + q] = kv.second[(size_t)t * 6 + q];
       ++J;
     }
+    src_of_point.resize(np);
+    for (int p = 0; p < np; ++p) src_of_point[p] = merged[cell[p]];
+    scale_source();
+  }
+  void scale_source() {
+    src_rate.assign((size_t)src_np * src_nt * 6, T(0));
+    for (int J = 0; J < src_np; ++J) {
+      double mu = (double)cjm[C_MU * N + src_idx[J]];
+      for (int t = 0; t < src_nt; ++t)
+        for (int q = 0; q < 6; ++q) {
+          size_t e = ((size_t)t * src_np + J) * 6 + q;
+          src_rate[e] = T(src_raw[e] * mu * src_vol[J]);
+        }
+    }
+  }
+
+  // Medium planes given directly as (mu, lambda, 1/rho) in FP64 (for gradient
+  // checks): the surface matrices and the source scaling follow them.
+  void set_medium_params(const double* mu, const double* lam, const double* b) {
+    for (long long p = 0; p < N; ++p) {
+      cjm[C_MU * N + p] = T(mu[p]);
+      cjm[C_LAM * N + p] = T(lam[p]);
+      cjm[C_B * N + p] = T(b[p]);
+    }
+    build_rdz(nullptr);
+    if (src_np) scale_source();
   }
 
   // ---------------------------------------------------------------------
@@ -622,6 +658,8 @@ struct Oracle {
 
 }  // namespace orc
 
+#include "cgfdm_adjoint.h"
+
 // ---------------------------------------------------------------------------
 // C ABI used by the tests (ctypes).
 // ---------------------------------------------------------------------------
@@ -804,5 +842,148 @@ int orc_step_index(void* hv) {
   int r = 0;
   DISPATCH(h, r = o->it);
   return r;
+}
+
+// ---- adjoint (FP64 oracle only) --------------------------------------------
+static double rnd_fill(std::vector<double>& v, unsigned long long& st) {
+  double s = 0;
+  for (auto& x : v) {
+    st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+    x = (double)((st >> 11) & ((1ULL << 53) - 1)) / (double)(1ULL << 53) - 0.5;
+    s += x * x;
+  }
+  return s;
+}
+
+// Dot-product test of the transposed operators on random vectors:
+// which = 0: stage right-hand side L with the F/B pattern of the current step;
+// which = 1: the whole RK4 step (no source). out2 = {<Op X, Y>, <X, Op^T Y>}.
+int orc_adjoint_dot(void* hv, int which, unsigned long long seed, double* out2) {
+  auto* h = (OrcHandle*)hv;
+  if (h->prec != 8) return 2;
+  auto* o = (orc::Oracle<double>*)h->p;
+  orc::Adjoint<double> ad(*o);
+  unsigned long long st = seed * 2654435761ULL + 1;
+  orc::AdjVec<double> X = ad.zeros(), Y = ad.zeros();
+  rnd_fill(X.W, st);
+  rnd_fill(Y.W, st);
+  for (auto& a : X.A) rnd_fill(a, st);
+  for (auto& a : Y.A) rnd_fill(a, st);
+  int FB[3] = {orc::FBTAB[o->it % 8][0], orc::FBTAB[o->it % 8][1], orc::FBTAB[o->it % 8][2]};
+  orc::AdjVec<double> OX, OTY = ad.zeros();
+  if (which == 0) {
+    OX = ad.rhs(FB, X);
+    ad.rhs_T(FB, Y, OTY, nullptr, nullptr, nullptr, nullptr);
+  } else {
+    orc::AdjVec<double> Xs[4];
+    ad.stages(FB, X, Xs);
+    // X' = X0 + sum b_s h_s
+    OX = X;
+    const double bw[4] = {orc::RK_B1, orc::RK_B2, orc::RK_B3, orc::RK_B4};
+    int F[3] = {FB[0], FB[1], FB[2]};
+    for (int s = 0; s < 4; ++s) {
+      orc::AdjVec<double> hs = ad.rhs(F, Xs[s]);
+      ad.axpy(bw[s], hs, OX);
+      F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2];
+    }
+    OTY = Y;
+    ad.step_T(o->it, X, OTY, nullptr, nullptr, nullptr);
+  }
+  out2[0] = ad.dot(OX, Y);
+  out2[1] = ad.dot(X, OTY);
+  return 0;
+}
+
+void orc_set_step_index(void* hv, int it) {
+  auto* h = (OrcHandle*)hv;
+  DISPATCH(h, o->it = it);
+}
+
+void orc_set_medium_params(void* hv, const double* mu, const double* lam, const double* b) {
+  auto* h = (OrcHandle*)hv;
+  DISPATCH(h, o->set_medium_params(mu, lam, b));
+}
+
+// Misfit J = 1/2 sum_it sum_s sum_{c<3} (W_c(station s) after step it - data[s][it][c])^2
+// over nsteps steps from the current state, and its gradient by the adjoint:
+// gmu, glam, gb per padded cell (may be null) and grate per input source point
+// [np][nt][6] in the units of the table given to set_source (may be null).
+// The oracle is left at the state after the forward run. Returns J (NaN on error).
+double orc_adjoint_misfit(void* hv, int nsteps, const double* data, double* gmu, double* glam, double* gb,
+                          double* grate) {
+  auto* h = (OrcHandle*)hv;
+  if (h->prec != 8) return std::nan("");
+  auto* o = (orc::Oracle<double>*)h->p;
+  orc::Adjoint<double> ad(*o);
+  const long long N = o->N;
+  const int it0 = o->it;
+  std::vector<orc::AdjVec<double>> Xpre;
+  std::vector<double> rec((size_t)o->ns * nsteps * 3);
+  std::vector<long long> scell(o->ns);
+  for (int s = 0; s < o->ns; ++s) scell[s] = o->at(o->st_ijk[3 * s], o->st_ijk[3 * s + 1], o->st_ijk[3 * s + 2]);
+  double J = 0;
+  for (int n = 0; n < nsteps; ++n) {
+    Xpre.push_back(ad.state());
+    o->step();
+    for (int s = 0; s < o->ns; ++s)
+      for (int c = 0; c < 3; ++c) {
+        double r = o->W[c * N + scell[s]] - data[((size_t)s * nsteps + n) * 3 + c];
+        rec[((size_t)s * nsteps + n) * 3 + c] = r;
+        J += 0.5 * r * r;
+      }
+  }
+  orc::AdjVec<double> endstate = ad.state();
+  std::vector<double> gm(N, 0.0), gl(N, 0.0), gbb(N, 0.0);
+  std::vector<double> gscaled((size_t)o->src_np * o->src_nt * 6, 0.0);
+  orc::AdjVec<double> mu = ad.zeros();
+  for (int n = nsteps - 1; n >= 0; --n) {
+    for (int s = 0; s < o->ns; ++s)
+      for (int c = 0; c < 3; ++c) mu.W[c * N + scell[s]] += rec[((size_t)s * nsteps + n) * 3 + c];
+    // state after the source injection of step n
+    ad.load(Xpre[n]);
+    o->it = it0 + n;
+    o->add_source();
+    orc::AdjVec<double> X0 = ad.state();
+    ad.step_T(it0 + n, X0, mu, gm.data(), gl.data(), gbb.data());
+    // source: W[3+q](c) -= ((m1 - m0) tw + m0) DT
+    if (o->src_np) {
+      float tmpT = (float)((o->it + 0.0f) * (float)o->DT);
+      int srcIt = int(tmpT / (float)o->src_dt);
+      if (srcIt + 1 < o->src_nt) {
+        float t1 = float(srcIt) * (float)o->src_dt;
+        double tw = (double)(float)((tmpT - t1) / (float)o->src_dt);
+        for (int p = 0; p < o->src_np; ++p)
+          for (int q = 0; q < 6; ++q) {
+            double gs = -o->DT * mu.W[(3 + q) * N + o->src_idx[p]];
+            gscaled[((size_t)srcIt * o->src_np + p) * 6 + q] += (1.0 - tw) * gs;
+            gscaled[((size_t)(srcIt + 1) * o->src_np + p) * 6 + q] += tw * gs;
+          }
+      }
+    }
+  }
+  // scaled rate = raw * mu(cell) / (J DH^3)
+  for (int p = 0; p < o->src_np; ++p) {
+    long long c = o->src_idx[p];
+    double muc = o->cjm[orc::C_MU * N + c];
+    for (int t = 0; t < o->src_nt; ++t)
+      for (int q = 0; q < 6; ++q) {
+        size_t e = ((size_t)t * o->src_np + p) * 6 + q;
+        gm[c] += gscaled[e] * o->src_raw[e] * o->src_vol[p];
+      }
+  }
+  if (grate)
+    for (size_t pi = 0; pi < o->src_of_point.size(); ++pi) {
+      int p = o->src_of_point[pi];
+      double muc = o->cjm[orc::C_MU * N + o->src_idx[p]];
+      for (int t = 0; t < o->src_nt; ++t)
+        for (int q = 0; q < 6; ++q)
+          grate[((size_t)pi * o->src_nt + t) * 6 + q] = gscaled[((size_t)t * o->src_np + p) * 6 + q] * muc * o->src_vol[p];
+    }
+  if (gmu) std::memcpy(gmu, gm.data(), sizeof(double) * N);
+  if (glam) std::memcpy(glam, gl.data(), sizeof(double) * N);
+  if (gb) std::memcpy(gb, gbb.data(), sizeof(double) * N);
+  ad.load(endstate);
+  o->it = it0 + nsteps;
+  return J;
 }
 }

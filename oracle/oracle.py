@@ -52,6 +52,12 @@ def lib():
         L.orc_rk_stage.argtypes = [i, i, ll, vp, vp, vp, vp]
         L.orc_step_index.restype = i
         L.orc_step_index.argtypes = [vp]
+        L.orc_adjoint_dot.restype = i
+        L.orc_adjoint_dot.argtypes = [vp, i, ctypes.c_ulonglong, vp]
+        L.orc_set_medium_params.argtypes = [vp, vp, vp, vp]
+        L.orc_set_step_index.argtypes = [vp, i]
+        L.orc_adjoint_misfit.restype = d
+        L.orc_adjoint_misfit.argtypes = [vp, i, vp, vp, vp, vp, vp]
         L.awo_create.restype = vp
         L.awo_create.argtypes = [i, i, i, i, i, d, d, i, vp, vp]
         L.awo_destroy.argtypes = [vp]
@@ -184,6 +190,38 @@ class Oracle:
         out = np.empty(n, np.float64)
         self.L.orc_get_face(self.h, f, comp, _ptr(out))
         return out
+
+
+def adjoint_dot(o, which, seed=1):
+    """(<Op X, Y>, <X, Op^T Y>) for random X, Y; which 0 = stage right-hand side
+    with the F/B pattern of the current step index, 1 = the whole RK4 step."""
+    out = np.zeros(2)
+    rc = lib().orc_adjoint_dot(o.h, int(which), int(seed), _ptr(out))
+    assert rc == 0
+    return out
+
+
+def set_step_index(o, it):
+    lib().orc_set_step_index(o.h, int(it))
+
+
+def set_medium_params(o, mu, lam, b):
+    """Overwrite (mu, lambda, 1/rho) per padded cell (FP64 arrays)."""
+    a = [np.ascontiguousarray(x, np.float64).ravel() for x in (mu, lam, b)]
+    lib().orc_set_medium_params(o.h, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]))
+
+
+def adjoint_misfit(o, nsteps, data, npts=0, nt_src=0):
+    """J = 1/2 sum (station velocity - data)^2 over nsteps steps from the current
+    state, and its adjoint gradient: dict(J, gmu, glam, gb, grate[npts][nt][6])."""
+    data = np.ascontiguousarray(data, np.float64)
+    N = o.N
+    gmu, glam, gb = np.zeros(N), np.zeros(N), np.zeros(N)
+    grate = np.zeros((max(npts, 1), max(nt_src, 1), 6))
+    J = lib().orc_adjoint_misfit(o.h, int(nsteps), _ptr(data), _ptr(gmu), _ptr(glam), _ptr(gb),
+                                 _ptr(grate) if npts else None)
+    return dict(J=J, gmu=gmu.reshape(o.shape), glam=glam.reshape(o.shape), gb=gb.reshape(o.shape),
+                grate=grate[:npts])
 
 
 def rk_stage(prec, irk, h, w, t, m):
