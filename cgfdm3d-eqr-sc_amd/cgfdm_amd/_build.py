@@ -14,8 +14,8 @@ LIB = os.path.join(PKG, "libcgfdm_amd.so")
 OBJ = os.path.join(ROOT, "_build")
 
 SOURCES = ["setup.hip", "step_fp32.hip", "step_fp64.hip", "step_fp16.hip", "step_bf16.hip", "aweno_fp32.hip",
-           "aweno_fp64.hip", "capi.cpp"]
-HEADERS = ["common.h", "kernels.h", "setup.h", "cgfdm_step_impl.h", "launch_impl.h", "aweno.h", "aweno_impl.h"]  # cgfdm_zmarch_disabled.h is not built
+           "aweno_fp64.hip", "adjoint_fp32.hip", "adjoint_fp64.hip", "capi.cpp"]
+HEADERS = ["common.h", "kernels.h", "setup.h", "cgfdm_step_impl.h", "launch_impl.h", "aweno.h", "aweno_impl.h", "adjoint.h", "adjoint_impl.h"]  # cgfdm_zmarch_disabled.h is not built
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -41,7 +41,8 @@ def build(verbose=True, jobs=None):
         results = list(ex.map(_compile, SOURCES))
     objs = [o for o, _ in results]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    # -z defs: an unresolved symbol fails the link instead of the first load
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp] + objs
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     os.replace(tmp, LIB)
     if verbose:

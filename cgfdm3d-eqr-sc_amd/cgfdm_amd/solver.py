@@ -41,7 +41,7 @@ EXPORTS = ["cgfdm_create", "cgfdm_destroy", "cgfdm_set_source", "cgfdm_set_stati
            "cgfdm_set_stream", "cgfdm_synchronize", "cgfdm_step_index", "cgfdm_get_stations",
            "cgfdm_get_pgv", "cgfdm_get_surface_displacement", "cgfdm_get_field", "cgfdm_set_field",
            "cgfdm_get_setup", "cgfdm_get_ranges", "cgfdm_timing", "cgfdm_device_bytes",
-           "cgfdm_last_error", "cgfdm_grid_ranges"]
+           "cgfdm_last_error", "cgfdm_grid_ranges", "cgfdm_forward_record", "cgfdm_adjoint"]
 
 
 def lib():
@@ -73,6 +73,8 @@ def lib():
     L.cgfdm_device_bytes.argtypes = [vp]
     L.cgfdm_last_error.restype = ctypes.c_char_p
     L.cgfdm_grid_ranges.argtypes = [i, i, i, vp, vp, vp]
+    L.cgfdm_forward_record.argtypes = [vp, i, i, vp]
+    L.cgfdm_adjoint.argtypes = [vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -217,6 +219,29 @@ class Solver:
         _check(self.L.cgfdm_timing(self.h, 1 if enable else 0, _ptr(out)))
         return dict(interior_ms=out[0], interior_n=int(out[1]), surface_ms=out[2], surface_n=int(out[3]),
                     step_ms=out[4], step_n=int(out[5]))
+
+    def forward_record(self, nsteps, checkpoint_every=100):
+        """Run nsteps from the current state keeping checkpoints for `adjoint`;
+        returns the station velocity [ns, nsteps, 3] (CGFDM scheme)."""
+        out = np.zeros((max(self.ns, 1), nsteps, 3), np.float64)
+        _check(self.L.cgfdm_forward_record(self.h, int(nsteps), int(checkpoint_every), _ptr(out)))
+        self._fr_n = nsteps
+        return out[: self.ns]
+
+    def adjoint(self, src):
+        """Adjoint of the last forward_record for the adjoint source
+        src[ns, nsteps, 3] = dJ/d(station velocity): gradients of J with respect to
+        mu, lambda, 1/rho per padded cell and to the moment-rate table given to
+        set_source ([npts, nt, 6])."""
+        NZ, NY, NX = self.shape
+        src = np.ascontiguousarray(src, np.float64)
+        gmu, glam, gb = (np.zeros(self.N) for _ in range(3))
+        npts = 0 if self.m.src_cells is None else len(self.m.src_cells)
+        nt = 0 if self.m.src_rate is None else self.m.src_rate.shape[1]
+        grate = np.zeros((max(npts, 1), max(nt, 1), 6))
+        _check(self.L.cgfdm_adjoint(self.h, _ptr(src), _ptr(gmu), _ptr(glam), _ptr(gb), _ptr(grate)))
+        return dict(gmu=gmu.reshape(self.shape), glam=glam.reshape(self.shape), gb=gb.reshape(self.shape),
+                    grate=grate[:npts])
 
     def device_bytes(self):
         return int(self.L.cgfdm_device_bytes(self.h))

@@ -9,12 +9,14 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "cgfdm_amd.h"
 #include "common.h"
 #include "aweno.h"
 #include "kernels.h"
+#include "adjoint.h"
 #include "setup.h"
 
 namespace {
@@ -57,6 +59,10 @@ struct SolverBase {
   virtual int get_field(int comp, double* out) = 0;
   virtual int set_field(int comp, const double* in) = 0;
   virtual int get_setup(int what, double* out) = 0;
+  virtual int forward_record(int, int, double*) { return fail(CGFDM_ERR_ARG, "the adjoint is available for the CGFDM scheme"); }
+  virtual int adjoint(const double*, double*, double*, double*, double*) {
+    return fail(CGFDM_ERR_ARG, "the adjoint is available for the CGFDM scheme");
+  }
   hipStream_t stream = nullptr;
   int it = 0;
   double ranges[8] = {0};
@@ -181,6 +187,22 @@ struct Solver : SolverBase {
   double src_dt = 0;
   long long* src_cell = nullptr;
   R* src_rate = nullptr;
+  // host copies for the source gradient: merged unscaled table (time-major),
+  // mu and 1/(J DH^3) per merged point, merged index of every input point
+  std::vector<double> src_raw, src_muv, src_vol;
+  std::vector<int> src_of_point;
+  // adjoint: checkpoints of the last forward_record and work buffers
+  int fr_n = 0, fr_ck = 1, fr_it0 = 0;
+  R* ck_store = nullptr;
+  R* rec_vel = nullptr;
+  R* adj_buf = nullptr;  // MU, OUT, G (9N each), Z (27N), Ls (9N), X3 (9N): 72N
+  R* adj_face = nullptr;
+  double* adj_grad = nullptr;  // gmu, glam, gb (N each)
+  double* adj_gsrc = nullptr;
+  R* adj_seg = nullptr;
+  R* adj_src = nullptr;
+  R* adj_end = nullptr;
+
   int init(const cgfdm_desc& dd, const float* coord, const float* medium) override {
     d = dd;
     NX = d.nx + 2 * cg::H;
@@ -322,9 +344,14 @@ struct Solver : SolverBase {
     if (src_np == 0) return CGFDM_OK;
     std::vector<long long> hc;
     std::vector<R> hr((size_t)src_np * nt * 6);
+    src_raw.assign((size_t)src_np * nt * 6, 0.0);
+    src_muv.assign(src_np, 0.0);
+    src_vol.assign(src_np, 0.0);
+    std::map<long long, int> merged;
     int Jp = 0;
     for (auto& kv : acc) {
       long long c = kv.first;
+      merged[c] = Jp;
       hc.push_back(c);
       R jv;
       TM mv;
@@ -334,10 +361,17 @@ struct Solver : SolverBase {
       if (sizeof(TM) == 8) mu = *(double*)(void*)&mv;
       double J = (double)jv;
       double sc = mu / (J * d.dh * d.dh * d.dh);  // mu/(J DH^3), stress in units of Z
+      src_muv[Jp] = mu;
+      src_vol[Jp] = 1.0 / (J * d.dh * d.dh * d.dh);
       for (int t = 0; t < nt; ++t)
-        for (int q = 0; q < 6; ++q) hr[((size_t)t * src_np + Jp) * 6 + q] = R(kv.second[(size_t)t * 6 + q] * sc);
+        for (int q = 0; q < 6; ++q) {
+          hr[((size_t)t * src_np + Jp) * 6 + q] = R(kv.second[(size_t)t * 6 + q] * sc);
+          src_raw[((size_t)t * src_np + Jp) * 6 + q] = kv.second[(size_t)t * 6 + q];
+        }
       ++Jp;
     }
+    src_of_point.resize(np);
+    for (int p = 0; p < np; ++p) src_of_point[p] = merged[cell[p]];
     if ((rc = dalloc(&src_cell, src_np))) return rc;
     if ((rc = dalloc(&src_rate, (long long)hr.size()))) return rc;
     HIPCHK(hipMemcpy(src_cell, hc.data(), sizeof(long long) * src_np, hipMemcpyHostToDevice));
@@ -367,7 +401,11 @@ struct Solver : SolverBase {
     return CGFDM_OK;
   }
 
-  int step(int n) override {
+  int step(int n) override { return step_impl(n, true); }
+
+  // observe = false: no station record, PGV or displacement update (used to
+  // recompute states for the adjoint)
+  int step_impl(int n, bool observe) {
     const int kend = cg::H + d.nz - (d.free_surface ? 3 : 0);
     for (int s = 0; s < n; ++s) {
       int fb[3] = {FBTAB[it % 8][0], FBTAB[it % 8][1], FBTAB[it % 8][2]};
@@ -415,7 +453,7 @@ struct Solver : SolverBase {
         cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream);
         if (timing_on) hipEventRecord(ev[ip].b, stream);
         if (d.free_surface) {
-          cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, pgv, dis};
+          cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, observe ? pgv : nullptr, dis};
           size_t fp = 0;
           if (timing_on) {
             fp = next_pair(1);
@@ -428,7 +466,7 @@ struct Solver : SolverBase {
         fb[1] = -fb[1];
         fb[2] = -fb[2];
       }
-      cg::launch_record<R>(A, N, ns, st_cell, st_rec, d.nt, it, stream);
+      if (observe) cg::launch_record<R>(A, N, ns, st_cell, st_rec, d.nt, it, stream);
       if (timing_on) hipEventRecord(ev[sp].b, stream);
       ++it;
     }
@@ -520,6 +558,208 @@ struct Solver : SolverBase {
       return CGFDM_OK;
     }
     return fail(CGFDM_ERR_ARG, "unknown setup product");
+  }
+
+  // ---- adjoint ------------------------------------------------------------
+  long long state_size() const {
+    long long n = 9 * N;
+    for (int f = 0; f < 6; ++f) n += 9 * pml.f[f].cnt;
+    return n;
+  }
+  int copy_state(R* dst, bool save) {
+    long long off = 0;
+    if (save) HIPCHK(hipMemcpyAsync(dst, A, sizeof(R) * 9 * N, hipMemcpyDeviceToDevice, stream));
+    else HIPCHK(hipMemcpyAsync(A, dst, sizeof(R) * 9 * N, hipMemcpyDeviceToDevice, stream));
+    off = 9 * N;
+    for (int f = 0; f < 6; ++f) {
+      const long long c = 9 * pml.f[f].cnt;
+      if (c == 0) continue;
+      if (save) HIPCHK(hipMemcpyAsync(dst + off, pml.f[f].a, sizeof(R) * c, hipMemcpyDeviceToDevice, stream));
+      else HIPCHK(hipMemcpyAsync(pml.f[f].a, dst + off, sizeof(R) * c, hipMemcpyDeviceToDevice, stream));
+      off += c;
+    }
+    return CGFDM_OK;
+  }
+  void release_adjoint() {
+    for (void* q : {(void*)ck_store, (void*)rec_vel, (void*)adj_buf, (void*)adj_face, (void*)adj_grad,
+                    (void*)adj_gsrc, (void*)adj_seg, (void*)adj_src, (void*)adj_end})
+      if (q) dfree(q);
+    ck_store = rec_vel = adj_buf = adj_face = adj_seg = adj_src = adj_end = nullptr;
+    adj_grad = adj_gsrc = nullptr;
+    fr_n = 0;
+  }
+  // source on A at step index `step` (addMoment.cpp:262-377); returns srcIt (-1 if none)
+  int inject_source(int step, float& tw) {
+    if (src_np <= 0) return -1;
+    float DTf = (float)d.dt;
+    float tmpT = ((float)step + 0.0f) * DTf;
+    int srcIt = int(tmpT / (float)src_dt);
+    if (srcIt + 1 >= src_nt) return -1;
+    float t1 = float(srcIt) * (float)src_dt;
+    tw = (tmpT - t1) / (float)src_dt;
+    cg::launch_source<R>(A, N, src_np, src_cell, src_rate, srcIt, R(tw), R(d.dt), stream);
+    return srcIt;
+  }
+  // RK stage states of the step at index `step` starting from A (after the
+  // source): X1 -> B, X2 -> C, X3 -> x3. Auxiliary states evolve in the face
+  // stage buffers; the face state buffers and A are left unchanged.
+  void stage_states(int step, R* x3) {
+    const int kend = cg::H + d.nz - (d.free_surface ? 3 : 0);
+    int fb[3] = {FBTAB[step % 8][0], FBTAB[step % 8][1], FBTAB[step % 8][2]};
+    R* ins[3] = {A, B, C};
+    R* outs[3] = {B, C, x3};
+    for (int irk = 0; irk < 3; ++irk) {
+      cg::StageArgs<R> a{};
+      a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.NX = NX; a.NY = NY; a.N = N; a.S = S;
+      a.kbeg = cg::H; a.kend = kend; a.stage = irk;
+      a.DT = R(d.dt); a.rDH = R(1.0 / d.dh);
+      a.win = ins[irk]; a.wout = outs[irk]; a.wm = A; a.wt = T; a.pml = pml;
+      cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream);
+      if (d.free_surface) {
+        cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, nullptr, dis};
+        cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], stream);
+      }
+      fb[0] = -fb[0]; fb[1] = -fb[1]; fb[2] = -fb[2];
+    }
+  }
+
+  int forward_record(int nsteps, int ck, double* rec) override {
+    if (nsteps < 1 || ck < 1) return fail(CGFDM_ERR_ARG, "nsteps and checkpoint_every must be positive");
+    if (d.coef_storage != 0) return fail(CGFDM_ERR_ARG, "the adjoint needs native coefficient storage");
+    release_adjoint();
+    int rc;
+    const long long ss = state_size();
+    const int nck = (nsteps + ck - 1) / ck;
+    if ((rc = dalloc(&ck_store, ss * nck))) return rc;
+    if ((rc = dalloc(&rec_vel, (long long)std::max(ns, 1) * nsteps * 3))) return rc;
+    fr_it0 = it;
+    fr_ck = ck;
+    for (int n = 0; n < nsteps; ++n) {
+      if (n % ck == 0 && (rc = copy_state(ck_store + (long long)(n / ck) * ss, true))) return rc;
+      if ((rc = step_impl(1, true))) return rc;
+      cg::launch_record_vel<R>(A, N, ns, st_cell, rec_vel, nsteps, n, stream);
+    }
+    HIPCHK(hipGetLastError());
+    fr_n = nsteps;
+    if (rec && ns > 0) {
+      std::vector<R> h((size_t)ns * nsteps * 3);
+      HIPCHK(hipMemcpyAsync(h.data(), rec_vel, sizeof(R) * h.size(), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      for (size_t q = 0; q < h.size(); ++q) rec[q] = (double)h[q];
+    }
+    return CGFDM_OK;
+  }
+
+  int adjoint(const double* src, double* gmu, double* glam, double* gb, double* grate) override {
+    if (fr_n <= 0) return fail(CGFDM_ERR_ARG, "no forward_record run to take the adjoint of");
+    if (ns > 0 && !src) return fail(CGFDM_ERR_ARG, "null adjoint source");
+    int rc;
+    const long long ss = state_size();
+    long long fcnt = 0;
+    for (int f = 0; f < 6; ++f) fcnt += 9 * pml.f[f].cnt;
+    if (!adj_buf) {
+      if ((rc = dalloc(&adj_buf, 72 * N))) return rc;
+      if ((rc = dalloc(&adj_face, std::max(3 * fcnt, 1LL)))) return rc;
+      if ((rc = dalloc(&adj_grad, 3 * N))) return rc;
+      if ((rc = dalloc(&adj_gsrc, std::max((long long)src_np * src_nt * 6, 1LL)))) return rc;
+      if ((rc = dalloc(&adj_seg, ss * fr_ck))) return rc;
+      if ((rc = dalloc(&adj_src, (long long)std::max(ns, 1) * fr_n * 3))) return rc;
+      if ((rc = dalloc(&adj_end, ss))) return rc;
+    }
+    R* MU = adj_buf;
+    R* OUT = adj_buf + 9 * N;
+    R* G = adj_buf + 18 * N;
+    R* Z = adj_buf + 27 * N;
+    R* Ls = adj_buf + 54 * N;
+    R* X3 = adj_buf + 63 * N;
+    HIPCHK(hipMemsetAsync(adj_buf, 0, sizeof(R) * 72 * N, stream));
+    HIPCHK(hipMemsetAsync(adj_face, 0, sizeof(R) * std::max(3 * fcnt, 1LL), stream));
+    HIPCHK(hipMemsetAsync(adj_grad, 0, sizeof(double) * 3 * N, stream));
+    HIPCHK(hipMemsetAsync(adj_gsrc, 0, sizeof(double) * std::max((long long)src_np * src_nt * 6, 1LL), stream));
+    if (ns > 0) {
+      std::vector<R> h((size_t)ns * fr_n * 3);
+      for (size_t q = 0; q < h.size(); ++q) h[q] = R(src[q]);
+      HIPCHK(hipMemcpyAsync(adj_src, h.data(), sizeof(R) * h.size(), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
+    cg::FaceAdj<R> fa[6];
+    {
+      long long off = 0;
+      for (int f = 0; f < 6; ++f) {
+        const long long c = 9 * pml.f[f].cnt;
+        fa[f] = cg::FaceAdj<R>{adj_face + off, adj_face + off + c, adj_face + off + 2 * c, pml.f[f].cnt};
+        off += 3 * c;
+      }
+    }
+    if ((rc = copy_state(adj_end, true))) return rc;
+    const int nck = (fr_n + fr_ck - 1) / fr_ck;
+    for (int seg = nck - 1; seg >= 0; --seg) {
+      const int n0 = seg * fr_ck, n1 = std::min(n0 + fr_ck, fr_n);
+      if ((rc = copy_state(ck_store + (long long)seg * ss, false))) return rc;
+      it = fr_it0 + n0;
+      for (int n = n0; n < n1; ++n) {
+        if ((rc = copy_state(adj_seg + (long long)(n - n0) * ss, true))) return rc;
+        if ((rc = step_impl(1, false))) return rc;
+      }
+      for (int n = n1 - 1; n >= n0; --n) {
+        cg::launch_adj_inject<R>(MU, N, ns, st_cell, adj_src, fr_n, n, stream);
+        if ((rc = copy_state(adj_seg + (long long)(n - n0) * ss, false))) return rc;
+        const int step = fr_it0 + n;
+        float tw = 0.f;
+        const int srcIt = inject_source(step, tw);
+        stage_states(step, X3);
+        const int* fb0 = FBTAB[step % 8];
+        for (int s = 3; s >= 0; --s) {
+          const int sg = (s % 2 == 0) ? 1 : -1;
+          cg::AdjArgs<R> a{};
+          a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.NX = NX; a.NY = NY; a.N = N; a.S = S;
+          a.fs = d.free_surface ? 1 : 0;
+          a.stage = s;
+          a.DT = R(d.dt);
+          a.rDH = R(1.0 / d.dh);
+          a.X = s == 0 ? A : s == 1 ? B : s == 2 ? C : X3;
+          a.mu = MU; a.out = OUT; a.G = G; a.Z = Z; a.Ls = Ls;
+          a.gmu = adj_grad; a.glam = adj_grad + N; a.gb = adj_grad + 2 * N;
+          a.pml = pml;
+          for (int f = 0; f < 6; ++f) a.fa[f] = fa[f];
+          if constexpr (std::is_same<R, TM>::value)
+            cg::launch_adj_stage<R, TM>(a, coef, jac, rdzdx, rdzdy, sg * fb0[0], sg * fb0[1], sg * fb0[2], stream);
+        }
+        if (srcIt >= 0)
+          cg::launch_adj_source<R>(OUT, N, src_np, src_cell, adj_gsrc, srcIt, (double)tw, (double)R(d.dt), stream);
+        std::swap(MU, OUT);
+        for (int f = 0; f < 6; ++f) std::swap(fa[f].mu, fa[f].out);
+      }
+    }
+    HIPCHK(hipGetLastError());
+    if ((rc = copy_state(adj_end, false))) return rc;
+    it = fr_it0 + fr_n;
+    std::vector<double> hg(3 * (size_t)N);
+    HIPCHK(hipMemcpyAsync(hg.data(), adj_grad, sizeof(double) * hg.size(), hipMemcpyDeviceToHost, stream));
+    std::vector<double> hs((size_t)src_np * src_nt * 6);
+    if (src_np > 0)
+      HIPCHK(hipMemcpyAsync(hs.data(), adj_gsrc, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    // scaled rate = raw mu(cell) / (J DH^3): the scaling carries mu of the source cell
+    std::vector<long long> hcell(src_np);
+    if (src_np > 0) HIPCHK(hipMemcpy(hcell.data(), src_cell, sizeof(long long) * src_np, hipMemcpyDeviceToHost));
+    for (int p = 0; p < src_np; ++p)
+      for (int t = 0; t < src_nt; ++t)
+        for (int q = 0; q < 6; ++q) {
+          const size_t e = ((size_t)t * src_np + p) * 6 + q;
+          hg[hcell[p]] += hs[e] * src_raw[e] * src_vol[p];
+        }
+    if (gmu) std::memcpy(gmu, hg.data(), sizeof(double) * N);
+    if (glam) std::memcpy(glam, hg.data() + N, sizeof(double) * N);
+    if (gb) std::memcpy(gb, hg.data() + 2 * N, sizeof(double) * N);
+    if (grate)
+      for (size_t pi = 0; pi < src_of_point.size(); ++pi) {
+        const int p = src_of_point[pi];
+        for (int t = 0; t < src_nt; ++t)
+          for (int q = 0; q < 6; ++q)
+            grate[((size_t)pi * src_nt + t) * 6 + q] = hs[((size_t)t * src_np + p) * 6 + q] * src_muv[p] * src_vol[p];
+      }
+    return CGFDM_OK;
   }
 };
 
@@ -863,6 +1103,16 @@ int cgfdm_destroy(cgfdm_solver* s) {
 
 #define SOLVER_OR_FAIL(s) \
   if (!(s) || !(s)->impl) return fail(CGFDM_ERR_ARG, "null solver")
+
+int cgfdm_forward_record(cgfdm_solver* s, int nsteps, int checkpoint_every, double* rec) {
+  SOLVER_OR_FAIL(s);
+  return s->impl->forward_record(nsteps, checkpoint_every, rec);
+}
+
+int cgfdm_adjoint(cgfdm_solver* s, const double* src, double* gmu, double* glam, double* gb, double* grate) {
+  SOLVER_OR_FAIL(s);
+  return s->impl->adjoint(src, gmu, glam, gb, grate);
+}
 
 int cgfdm_set_source(cgfdm_solver* s, int npts, int nt, double dt, const long long* cell, const float* rate) {
   SOLVER_OR_FAIL(s);

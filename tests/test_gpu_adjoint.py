@@ -1,0 +1,89 @@
+"""HIP adjoint of the CGFDM step against the oracle's discrete adjoint (MI355X).
+
+The oracle adjoint is pinned by the dot-product identity and by central finite
+differences (tests/test_adjoint_oracle.py). Here the GPU gradients of
+J = 1/2 |station velocity|^2 with respect to mu, lambda, 1/rho and the
+moment-rate table must match it: FP64 to 1e-10 of the largest entry, FP32 to
+1e-3 (the FP32 forward/adjoint accumulate rounding over the run).
+"""
+import numpy as np
+import pytest
+
+from cgfdm_amd import model as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def solver_mod():
+    from cgfdm_amd import solver
+    return solver
+
+
+CASES = {
+    "pml_fs_curvilinear": dict(nx=20, ny=18, nz=14, npml=4, hills=2, free_surface=True),
+    "pml_no_fs": dict(nx=18, ny=16, nz=12, npml=3, hills=0, free_surface=False),
+    "no_pml_fs": dict(nx=16, ny=14, nz=12, npml=0, hills=2, free_surface=True),
+}
+
+
+def run(name, prec, oracle_mod, solver_mod, nst=12, ck=5):
+    m = M.small_config(nt=nst, **CASES[name])
+    print(f"adjoint case {name} prec={prec} steps={nst} checkpoint={ck}", flush=True)
+    g = solver_mod.Solver(m, precision=prec)
+    rec = g.forward_record(nst, ck)
+    gr = g.adjoint(rec)
+    o = oracle_mod.Oracle(m, prec=8, threads=8)
+    r = oracle_mod.adjoint_misfit(o, nst, np.zeros((len(m.stations), nst, 3)), npts=len(m.src_cells),
+                                  nt_src=m.src_rate.shape[1])
+    return m, g, rec, gr, r
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp64_adjoint_matches_oracle(name, oracle_mod, solver_mod):
+    m, g, rec, gr, r = run(name, 8, oracle_mod, solver_mod)
+    assert 0.5 * (rec ** 2).sum() == pytest.approx(r["J"], rel=1e-10)
+    for k in ("gmu", "glam", "gb", "grate"):
+        scale = np.abs(r[k]).max()
+        assert scale > 0
+        np.testing.assert_allclose(gr[k], r[k], rtol=0, atol=1e-10 * scale, err_msg=k)
+
+
+def test_fp32_adjoint_matches_oracle(oracle_mod, solver_mod):
+    m, g, rec, gr, r = run("pml_fs_curvilinear", 4, oracle_mod, solver_mod)
+    for k in ("gmu", "glam", "gb", "grate"):
+        err = np.linalg.norm(gr[k] - r[k]) / np.linalg.norm(r[k])
+        assert err < 1e-3, (k, err)
+
+
+def test_adjoint_leaves_the_forward_state(oracle_mod, solver_mod):
+    """After forward_record + adjoint the solver is at the end of the forward run
+    and continues exactly like a plain run."""
+    m = M.small_config(nt=20, **CASES["pml_fs_curvilinear"])
+    a = solver_mod.Solver(m, precision=8)
+    rec = a.forward_record(10, 3)
+    a.adjoint(rec)
+    a.step(5)
+    b = solver_mod.Solver(m, precision=8)
+    b.step(15)
+    assert a.it == b.it == 15
+    np.testing.assert_array_equal(a.wavefield(), b.wavefield())
+
+
+def test_autograd_wrapper_on_the_hip_backend(oracle_mod):
+    """torch gradients through the HIP adjoint equal those through the oracle."""
+    import torch
+    from cgfdm_amd import autograd as A
+    from test_autograd_wrapper import OracleBackend
+    nst = 8
+    m = M.small_config(nx=12, ny=11, nz=10, npml=3, hills=2, free_surface=True, nt=nst)
+    w = torch.tensor(np.random.default_rng(3).standard_normal((len(m.stations), nst, 3)))
+    grads = []
+    for backend in (None, lambda mm, prec: OracleBackend(mm, prec, oracle_mod)):
+        ts = [torch.tensor(m.medium[c], dtype=torch.float64, requires_grad=True) for c in range(3)]
+        rate = torch.tensor(m.src_rate, dtype=torch.float64, requires_grad=True)
+        loss = (A.seismograms(*ts, rate, m, nst, precision=8, checkpoint_every=3, backend=backend) * w).sum()
+        loss.backward()
+        grads.append([t.grad.numpy() for t in ts] + [rate.grad.numpy()])
+    for a, b in zip(*grads):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-10 * np.abs(b).max())
