@@ -111,6 +111,22 @@ int cgfdm_get_setup(cgfdm_solver* s, int what, double* out);
 /* [Hmin, Hmax, Vs_min, Vs_max, Vp_min, Vp_max, rho_min, rho_max] (calc_CFL.cpp:316-340). */
 int cgfdm_get_ranges(cgfdm_solver* s, double* out8);
 
+/* Adjoint (CGFDM scheme, coef_storage 0). The reference has no adjoint; these serve the
+ * waveform-gradient workflow (gradients of a misfit of the station velocities with respect to the
+ * medium and the source table).
+ * cgfdm_forward_record: run nsteps steps from the current state (station records, PGV and
+ * displacement update as cgfdm_step does), keep a checkpoint of the state (W and PML
+ * auxiliaries) every checkpoint_every steps, and return the station velocity
+ * rec[ns][nsteps][3] in the wavefield precision. */
+int cgfdm_forward_record(cgfdm_solver* s, int nsteps, int checkpoint_every, double* rec);
+/* cgfdm_adjoint: discrete adjoint of the last cgfdm_forward_record for the adjoint source
+ * src[ns][nsteps][3] = dJ/d rec. Segments between checkpoints are recomputed, the RK4 stages
+ * of every step are recomputed from its start, and the transposed stages run backwards.
+ * Outputs (any may be NULL): dJ/dmu, dJ/dlambda, dJ/d(1/rho) per padded cell
+ * (double[NZ][NY][NX]), and dJ/d(table given to cgfdm_set_source) as double[npts][nt][6].
+ * The solver is left at the end of the forward run. */
+int cgfdm_adjoint(cgfdm_solver* s, const double* src, double* gmu, double* glam, double* gb, double* grate);
+
 /* Per-kernel timing with HIP events on the solver stream. enable = 1 starts
  * accumulation (and clears it), 0 stops. out[0..5] = {interior ms, interior
  * launches, surface ms, surface launches, step ms, steps}; out may be NULL.
