@@ -13,6 +13,9 @@ recorded on the solver's stream inside the timed region.
 
 `--scheme aweno` measures the ME-AWENO step instead (the 512 x 512 x 256 terrain
 grid with a sharp low-velocity basin, TVD-RK3, sponge) and prints its own line.
+`--adjoint` measures forward + adjoint against the plain forward on the adjoint
+workload (256 x 256 x 128, terrain, layered + random medium, 16 x 8 finite
+fault, 64 stations, checkpoints) and prints its own line.
 """
 import argparse
 import json
@@ -90,6 +93,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--scheme", choices=["cgfdm", "aweno"], default="cgfdm")
+    ap.add_argument("--adjoint", action="store_true")
+    ap.add_argument("--checkpoint", type=int, default=20)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -105,6 +110,8 @@ def main():
 
     if args.scheme == "aweno":
         return bench_aweno(args)
+    if args.adjoint:
+        return bench_adjoint(args)
     t0 = time.perf_counter()
     hmin_gpu = lambda c, md: S.grid_ranges(args.nx, args.ny, args.nz, c, md)["hmin"]
     m = M.config2(nt=args.warmup + args.steps + 1, nx=args.nx, ny=args.ny, nz=args.nz, hmin_fn=hmin_gpu)
@@ -197,6 +204,53 @@ def main():
                      "kernel": "cg_cell", "launch_ms": launch_ms,
                      "alg_bytes_per_cell": stage_bytes(args.coef), "cells_per_launch": cells_launch},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args):
+    """forward_record + adjoint vs plain forward steps, FP32, same workload."""
+    import numpy as np
+    import torch
+    from cgfdm_amd import model as M
+    from cgfdm_amd import solver as S
+
+    nx, ny, nz = 256, 256, 128
+    hmin_gpu = lambda c, md: S.grid_ranges(nx, ny, nz, c, md)["hmin"]
+    m = M.config5(nt=args.steps + args.warmup + 1, hmin_fn=hmin_gpu)
+    log(f"adjoint model {m.nx}x{m.ny}x{m.nz}, {len(m.src_cells)} fault points, {len(m.stations)} stations")
+    g = S.Solver(m, precision=4)
+    g.step(args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.synchronize()
+    t_fwd = time.perf_counter() - t0
+    g.close()
+    torch.cuda.empty_cache()
+    a = S.Solver(m, precision=4)
+    a.step(args.warmup)
+    a.synchronize()
+    t0 = time.perf_counter()
+    rec = a.forward_record(args.steps, args.checkpoint)
+    t_rec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grads = a.adjoint(rec)
+    t_adj = time.perf_counter() - t0
+    ok = all(np.isfinite(grads[k]).all() for k in ("gmu", "glam", "gb"))
+    a.close()
+    ratio = (t_rec + t_adj) / t_fwd
+    log(f"forward {t_fwd:.3f}s, forward_record {t_rec:.3f}s, adjoint {t_adj:.3f}s, ratio {ratio:.2f}")
+    out = {
+        "metric": METRIC, "value": m.cells * args.steps / (t_rec + t_adj) / 1e9, "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": (t_rec + t_adj) * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"CGFDM forward+adjoint {m.nx}x{m.ny}x{m.nz} h=100m, 4 hills, 4 layers + 5% random, "
+                               f"PML 20, 16x8 finite fault, {len(m.stations)} stations, checkpoint every "
+                               f"{args.checkpoint} steps", "cells": m.cells},
+        "forward_gcells": m.cells * args.steps / t_fwd / 1e9,
+        "cost_ratio_fwd_adj_vs_fwd": ratio, "gradients_finite": ok,
     }
     print(json.dumps(out), flush=True)
 

@@ -45,6 +45,7 @@ class Model:
     stations: Optional[np.ndarray] = None    # int32 [ns, 3] padded (i, j, k)
     name: str = ""
     terrain: Optional[np.ndarray] = None     # float32 [NY, NX] surface elevation
+    src_mt: Optional[np.ndarray] = None      # [np, 6] moment tensor * area per finite-fault point
 
     @property
     def shape_padded(self):
@@ -215,6 +216,77 @@ def point_source(model: Model, i, j, k, strike, dip, rake, m0, fc, nt_src=None):
     return model
 
 
+def nearest_cell(model: Model, x, y, z):
+    """Padded (i, j, k) of the grid point next to (x, y, z) in model coordinates
+    (srcCoord2Index, multiSource.cpp:173-314, for a local Cartesian point): i, j
+    from x, y on the regular horizontal grid, k the interior cell of that column
+    whose z is closest."""
+    NZ, NY, NX = model.shape_padded
+    x0 = float(model.coord[0, 0, 0, HALO])
+    y0 = float(model.coord[1, 0, HALO, 0])
+    i = int(round((x - x0) / model.dh)) + HALO
+    j = int(round((y - y0) / model.dh)) + HALO
+    i = min(max(i, HALO), NX - HALO - 1)
+    j = min(max(j, HALO), NY - HALO - 1)
+    col = model.coord[2, HALO:NZ - HALO, j, i].astype(np.float64)
+    k = int(np.argmin(np.abs(col - z))) + HALO
+    return i, j, k
+
+
+def von_karman_slip(nl, nw, dl, dw, a_s, a_d, hurst, mean, seed):
+    """Seeded random slip on an nw x nl subfault grid with a von Karman spectrum
+    P(k) ~ (1 + k_s^2 a_s^2 + k_d^2 a_d^2)^-(H+1), shifted and scaled to the
+    given mean and clipped at zero."""
+    rng = np.random.default_rng(seed)
+    ks = np.fft.fftfreq(nl, d=dl) * 2 * math.pi
+    kd = np.fft.fftfreq(nw, d=dw) * 2 * math.pi
+    KD, KS = np.meshgrid(kd, ks, indexing="ij")
+    amp = (1.0 + (KS * a_s) ** 2 + (KD * a_d) ** 2) ** (-(hurst + 1.0) / 2.0)
+    noise = np.fft.fft2(rng.standard_normal((nw, nl)))
+    f = np.real(np.fft.ifft2(noise * amp))
+    f = (f - f.mean()) / (f.std() + 1e-30)
+    slip = mean * (1.0 + 0.5 * f)
+    slip = np.clip(slip, 0.0, None)
+    return slip * (mean / slip.mean())
+
+
+def finite_fault(model: Model, x_top, y_top, depth_top, strike, dip, rake, length, width, nl, nw, slip, vr,
+                 rise, hypo=(0.5, 0.5), nt_src=None):
+    """Planar finite fault of nl x nw subfaults (one point each), the input of
+    solveMomentRate (multiSource.cpp:450-535): rate[p, t] = M(strike, dip, rake)
+    * area * slip_rate_p(t), with a triangular slip rate of duration `rise`
+    starting at the rupture time distance(hypocentre) / vr and integrating to
+    the subfault slip. x_top, y_top: centre of the top edge; depth below z = 0.
+    Sets model.src_cells / src_rate / src_dt; returns M * area per point [np, 6]
+    (for slip-rate gradients)."""
+    d2r = math.pi / 180.0
+    s, d = strike * d2r, dip * d2r
+    u_s = np.array([math.sin(s), math.cos(s), 0.0])                       # along strike
+    u_d = np.array([math.cos(s) * math.cos(d), -math.sin(s) * math.cos(d), -math.sin(d)])  # down dip
+    dl, dw = length / nl, width / nw
+    top = np.array([x_top, y_top, -depth_top])
+    hyp = top + (hypo[0] - 0.5) * length * u_s + hypo[1] * width * u_d
+    nt_src = nt_src or model.nt + 2
+    t = np.arange(nt_src) * model.dt
+    M = moment_tensor(strike, dip, rake)
+    cells, rates, mts = [], [], []
+    slip = np.broadcast_to(np.asarray(slip, np.float64), (nw, nl))
+    for b in range(nw):
+        for a in range(nl):
+            P = top + ((a + 0.5) * dl - 0.5 * length) * u_s + (b + 0.5) * dw * u_d
+            i, j, k = nearest_cell(model, P[0], P[1], P[2])
+            tr = float(np.linalg.norm(P - hyp)) / vr
+            sr = triangle_rate(t, tr, rise, float(slip[b, a]))
+            area = dl * dw
+            cells.append(model.cell_index(i, j, k))
+            rates.append(M[None, :] * area * sr[:, None])
+            mts.append(M * area)
+    model.src_cells = np.array(cells, np.int64)
+    model.src_rate = np.ascontiguousarray(np.array(rates), dtype=np.float32)
+    model.src_dt = model.dt
+    return np.array(mts)
+
+
 # ---------------------------------------------------------------------------
 # time step from the CFL estimate
 # ---------------------------------------------------------------------------
@@ -315,6 +387,31 @@ def config3(nt=4000, nx=512, ny=512, nz=256, dh=100.0, seed=1002, nhills=8, hmin
     point_source(m, i, j, k, 30.0, 60.0, 90.0, 1e16, 2.0)
     nside = 20 if min(nx, ny) - 40 >= 20 else max(2, min(nx, ny) - 40)
     m.stations = surface_station_grid(nx, ny, nz, 20, nside)
+    return m
+
+
+def config5(nt=2000, nx=256, ny=256, nz=128, dh=100.0, npml=20, seed=1005, hmin_fn=None):
+    """Adjoint workload: 256 x 256 x 128 at h = 100 m, 4 seeded Gaussian hills,
+    4-layer medium with a 5% Gaussian random perturbation (1 km correlation),
+    20-layer PML, a 16 x 8 subfault finite fault (16 x 8 km, strike 30, dip 60,
+    rake 90, top at 2 km depth, uniform 1 m slip, rupture 0.8 Vs, 1 s rise),
+    64 surface stations on an 8 x 8 grid."""
+    depth = nz * dh
+    terrain = gaussian_hills(nx, ny, dh, npml, 4, seed)
+    coord = terrain_coord(nx, ny, nz, dh, terrain, depth)
+    vp = np.array([3000.0, 4500.0, 6000.0, 7000.0])
+    med = layered_medium(coord, terrain, [0.0, 2000.0, 6000.0, 10000.0], vp / 1.73, vp,
+                         [2300.0, 2500.0, 2700.0, 3000.0])
+    delta = gaussian_random_field(med.shape[1:], dh, 1000.0, 0.05, seed + 7)
+    med[0] *= 1.0 + delta
+    med[1] *= 1.0 + delta
+    hmin = (hmin_fn or (lambda c, md: hmin_point_plane(c)))(coord, med)
+    dt = 0.5 * hmin / float(med[1].max())
+    m = Model(nx, ny, nz, dh, dt, nt, npml, coord, med, True, name="config5", terrain=terrain)
+    xc, yc = (nx // 2) * dh, (ny // 2) * dh
+    vs_src = float(np.median(med[0]))
+    m.src_mt = finite_fault(m, xc, yc, 2000.0, 30.0, 60.0, 90.0, 16000.0, 8000.0, 16, 8, 1.0, 0.8 * vs_src, 1.0)
+    m.stations = surface_station_grid(nx, ny, nz, npml, 8)
     return m
 
 
