@@ -14,21 +14,35 @@ namespace aw {
 
 using cg::H;
 
+// Reciprocal used for the WENO weights: the hardware reciprocal (1 ulp) in
+// FP32, an exact division in FP64.
+__device__ __forceinline__ float wrcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double wrcp(double x) { return 1.0 / x; }
+
 // WENO5-JS value at x_{i+1/2} from u1..u5 = u(i-2..i+2)
-// (alternative_flux_FD.cpp:43-59). The weights are d_k / (beta_k + 1e-6)^2 with
-// d = (5/16, 10/16, 1/16), evaluated in the working precision.
+// (alternative_flux_FD.cpp:43-59). The smoothness indicators are written as
+// the sums of squares they expand to, e.g. (10 u3^2 - 31 u3 u4 + 25 u4^2 +
+// 11 u3 u5 - 19 u4 u5 + 4 u5^2) / 3 = 13/12 (u3 - 2 u4 + u5)^2 +
+// 1/4 (3 u3 - 4 u4 + u5)^2; the weights are d_k / (beta_k + 1e-6)^2 with
+// d = (5/16, 10/16, 1/16), normalised.
 template <class R>
 __device__ __forceinline__ R weno5(R u1, R u2, R u3, R u4, R u5) {
-  const R third = R(0.3333333333);
-  R b1 = third * (R(10) * u3 * u3 - R(31) * u3 * u4 + R(25) * u4 * u4 + R(11) * u3 * u5 - R(19) * u4 * u5 + R(4) * u5 * u5);
-  R b2 = third * (R(4) * u2 * u2 - R(13) * u2 * u3 + R(13) * u3 * u3 + R(5) * u2 * u4 - R(13) * u3 * u4 + R(4) * u4 * u4);
-  R b3 = third * (R(4) * u1 * u1 - R(19) * u1 * u2 + R(25) * u2 * u2 + R(11) * u1 * u3 - R(31) * u2 * u3 + R(10) * u3 * u3);
+  // the reference scales the expanded form by the literal 0.3333333333, i.e.
+  // the sums of squares by 0.9999999999 (exactly 1 in FP32)
+  const R kb = R(3.0 * 0.3333333333);
+  const R c13 = kb * R(13.0 / 12.0), c14 = kb * R(0.25);
+  const R a1 = u3 - R(2) * u4 + u5, e1 = R(3) * u3 - R(4) * u4 + u5;
+  const R a2 = u2 - R(2) * u3 + u4, e2 = u2 - u4;
+  const R a3 = u1 - R(2) * u2 + u3, e3 = u1 - R(4) * u2 + R(3) * u3;
+  const R b1 = c13 * a1 * a1 + c14 * e1 * e1;
+  const R b2 = c13 * a2 * a2 + c14 * e2 * e2;
+  const R b3 = c13 * a3 * a3 + c14 * e3 * e3;
   const R eps = R(1e-6);
-  R q1 = b1 + eps, q2 = b2 + eps, q3 = b3 + eps;
-  R w1 = R(0.3125) / (q1 * q1);
-  R w2 = R(0.625) / (q2 * q2);
-  R w3 = R(0.0625) / (q3 * q3);
-  R tot = R(1) / (w1 + w2 + w3);
+  const R q1 = b1 + eps, q2 = b2 + eps, q3 = b3 + eps;
+  const R w1 = R(0.3125) * wrcp(q1 * q1);
+  const R w2 = R(0.625) * wrcp(q2 * q2);
+  const R w3 = R(0.0625) * wrcp(q3 * q3);
+  const R tot = wrcp(w1 + w2 + w3);
   return (w1 * tot) * (R(0.375) * u3 + R(0.75) * u4 - R(0.125) * u5) +
          (w2 * tot) * (R(-0.125) * u2 + R(0.75) * u3 + R(0.375) * u4) +
          (w3 * tot) * (R(0.375) * u1 - R(1.25) * u2 + R(1.875) * u3);
