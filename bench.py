@@ -11,8 +11,9 @@ Inputs are resident in HBM before the timed region starts.
 Prints one JSON line (see the keys below). Per-kernel times come from HIP events
 recorded on the solver's stream inside the timed region.
 
-`--scheme aweno` measures the ME-AWENO step instead (the 512 x 512 x 256 terrain
-grid with a sharp low-velocity basin, TVD-RK3, sponge) and prints its own line.
+`--scheme aweno` (the default while the CGFDM step kernels are switched off in
+the library) measures the ME-AWENO step instead (the 512 x 512 x 256 terrain grid
+with a sharp low-velocity basin, TVD-RK3, sponge) and prints its own line.
 `--adjoint` measures forward + adjoint against the plain forward on the adjoint
 workload (256 x 256 x 128, terrain, layered + random medium, 16 x 8 finite
 fault, 64 stations, checkpoints) and prints its own line.
@@ -92,7 +93,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--scheme", choices=["cgfdm", "aweno"], default="cgfdm")
+    # default ME-AWENO: the CGFDM step kernels are switched off in the library
+    # (unexplained GPU memory fault on their first run); --scheme cgfdm fails loudly
+    ap.add_argument("--scheme", choices=["cgfdm", "aweno"], default="aweno")
     ap.add_argument("--adjoint", action="store_true")
     ap.add_argument("--checkpoint", type=int, default=20)
     args = ap.parse_args()
@@ -305,10 +308,14 @@ def bench_aweno(args):
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        v, secs = cpu_baseline(m, args.cpu_steps, threads, scheme="aweno")
+        # bounded sample: the same workload built at half the extent per axis
+        # (1/8 of the cells, about 10 s per step on 16 host threads)
+        mc = M.config3(nt=args.cpu_steps + 2, nx=args.nx // 2, ny=args.ny // 2, nz=args.nz // 2)
+        v, secs = cpu_baseline(mc, args.cpu_steps, threads, scheme="aweno")
         cpu = {"value": v, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
                "sample": f"CPU oracle (C++ restatement, FP32, OpenMP) of the ME-AWENO step on the same "
-                         f"{m.nx}x{m.ny}x{m.nz} grid, {args.cpu_steps} full steps ({secs:.1f}s)"}
+                         f"workload at half extent per axis ({mc.nx}x{mc.ny}x{mc.nz}: terrain, basin, "
+                         f"sponge, source, stations), {args.cpu_steps} full steps ({secs:.1f}s)"}
     out = {
         "metric": METRIC, "value": gcells, "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",

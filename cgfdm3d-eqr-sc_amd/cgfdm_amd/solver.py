@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_PKG, "libcgfdm_amd.so")
 SCHEME_CGFDM = 0
 SCHEME_AWENO = 1
 
-ERRORS = {0: "ok", 2: "invalid argument", 3: "HIP error", 100: "no params.json", 1001: "parse error",
+ERRORS = {0: "ok", 2: "invalid argument", 3: "HIP error", 4: "switched off", 100: "no params.json", 1001: "parse error",
           110: "CFL violated", 130: "PML too thick", 180: "bad source", 10001: "allocation failure"}
 
 

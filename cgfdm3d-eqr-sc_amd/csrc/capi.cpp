@@ -28,6 +28,16 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+// The CGFDM step kernels (cg_cell, cg_surface, cg_add_source, cg_record) and the
+// adjoint built on them are switched off: their first MI355X runs (FP64, 24x20x16,
+// PML 4, free surface) ended in a GPU memory fault whose cause has not been shown,
+// while the same source runs clean in a host emulation with guard pages and
+// reproduces the FP64 oracle. Grid setup, sources, stations and the setup
+// products stay available; stepping returns CGFDM_ERR_DISABLED.
+constexpr bool kCgfdmStepEnabled = false;
+const char* const kCgfdmStepOff =
+    "the CGFDM step kernels are switched off in this build (unexplained GPU memory fault on their first run)";
+
 #define HIPCHK(x)                                                                                  \
   do {                                                                                             \
     hipError_t e_ = (x);                                                                           \
@@ -406,6 +416,7 @@ struct Solver : SolverBase {
   // observe = false: no station record, PGV or displacement update (used to
   // recompute states for the adjoint)
   int step_impl(int n, bool observe) {
+    if (!kCgfdmStepEnabled) return fail(CGFDM_ERR_DISABLED, kCgfdmStepOff);
     const int kend = cg::H + d.nz - (d.free_surface ? 3 : 0);
     for (int s = 0; s < n; ++s) {
       int fb[3] = {FBTAB[it % 8][0], FBTAB[it % 8][1], FBTAB[it % 8][2]};
@@ -651,6 +662,7 @@ struct Solver : SolverBase {
   }
 
   int adjoint(const double* src, double* gmu, double* glam, double* gb, double* grate) override {
+    if (!kCgfdmStepEnabled) return fail(CGFDM_ERR_DISABLED, kCgfdmStepOff);
     if (fr_n <= 0) return fail(CGFDM_ERR_ARG, "no forward_record run to take the adjoint of");
     if (ns > 0 && !src) return fail(CGFDM_ERR_ARG, "null adjoint source");
     int rc;

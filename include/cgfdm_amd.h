@@ -31,6 +31,8 @@ extern "C" {
 #define CGFDM_ERR_PML 130        /* nPML >= local n (pml_deriv.cpp:55-85) */
 #define CGFDM_ERR_SOURCE 180     /* inconsistent source description (modelChecking.cpp:23-50) */
 #define CGFDM_ERR_ALLOC 10001    /* device allocation failure (wave_deriv.cpp:49-53) */
+#define CGFDM_ERR_DISABLED 4     /* code path switched off in this build (no reference counterpart):
+                                    CGFDM stepping, forward_record and adjoint */
 
 #define CGFDM_SCHEME_CGFDM 0     /* Makefile_CGFDM: MacCormack + RK4 + CFS-PML + traction image */
 #define CGFDM_SCHEME_AWENO 1     /* Makefile_SCFDM: WENO/LF alternative flux + TVD-RK3 + sponge */

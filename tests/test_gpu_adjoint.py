@@ -11,7 +11,9 @@ import pytest
 
 from cgfdm_amd import model as M
 
-pytestmark = pytest.mark.gpu
+from conftest import CGFDM_STEP_OFF
+
+pytestmark = [pytest.mark.gpu, CGFDM_STEP_OFF]
 
 
 @pytest.fixture(scope="module")
