@@ -11,9 +11,8 @@ Inputs are resident in HBM before the timed region starts.
 Prints one JSON line (see the keys below). Per-kernel times come from HIP events
 recorded on the solver's stream inside the timed region.
 
-`--scheme aweno` (the default while the CGFDM step kernels are switched off in
-the library) measures the ME-AWENO step instead (the 512 x 512 x 256 terrain grid
-with a sharp low-velocity basin, TVD-RK3, sponge) and prints its own line.
+`--scheme aweno` measures the ME-AWENO step instead (the 512 x 512 x 256 terrain
+grid with a sharp low-velocity basin, TVD-RK3, sponge) and prints its own line.
 `--adjoint` measures forward + adjoint against the plain forward on the adjoint
 workload (256 x 256 x 128, terrain, layered + random medium, 16 x 8 finite
 fault, 64 stations, checkpoints) and prints its own line.
@@ -90,14 +89,14 @@ def main():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--coef", type=int, default=0, help="0 fp32, 1 fp16, 2 bf16 coefficient storage")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    # default ME-AWENO: the CGFDM step kernels are switched off in the library
-    # (unexplained GPU memory fault on their first run); --scheme cgfdm fails loudly
-    ap.add_argument("--scheme", choices=["cgfdm", "aweno"], default="aweno")
+    ap.add_argument("--scheme", choices=["cgfdm", "aweno"], default="cgfdm")
     ap.add_argument("--adjoint", action="store_true")
     ap.add_argument("--checkpoint", type=int, default=20)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (reference for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -111,6 +110,8 @@ def main():
     from cgfdm_amd import model as M
     from cgfdm_amd import solver as S
 
+    if args.copy_calibration:
+        return copy_calibration()
     if args.scheme == "aweno":
         return bench_aweno(args)
     if args.adjoint:
@@ -176,10 +177,14 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        v, secs = cpu_baseline(m, args.cpu_steps, threads)
+        # bounded sample: the same workload built at half the extent per axis
+        # (1/8 of the cells, a few seconds per step on 16 host threads)
+        mc = M.config2(nt=args.cpu_steps + 2, nx=args.nx // 2, ny=args.ny // 2, nz=args.nz // 2)
+        v, secs = cpu_baseline(mc, args.cpu_steps, threads)
         cpu = {"value": v, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
-               "sample": f"CPU oracle (C++ restatement, FP32, OpenMP) on the same {m.nx}x{m.ny}x{m.nz} grid, "
-                         f"{args.cpu_steps} full steps ({secs:.1f}s)"}
+               "sample": f"CPU oracle (C++ restatement, FP32, OpenMP) of the CGFDM step on the same workload "
+                         f"at half extent per axis ({mc.nx}x{mc.ny}x{mc.nz}: terrain, layered + random medium, "
+                         f"PML 20, free surface, source, stations), {args.cpu_steps} full steps ({secs:.1f}s)"}
         log(f"cpu baseline {v:.4f} Gcell/s")
 
     dtype = {0: "f32", 1: "f32 (fp16 coefficients)", 2: "f32 (bf16 coefficients)"}[args.coef]
@@ -256,6 +261,21 @@ def bench_adjoint(args):
         "cost_ratio_fwd_adj_vs_fwd": ratio, "gradients_finite": ok,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration():
+    """Five copies of 1 GiB between two device buffers: the counter reading per
+    copy dispatch divided by 2^30 B read (and written) is the scale factor of
+    FETCH_SIZE (WRITE_SIZE) for this GPU."""
+    import torch
+    n = 1 << 28  # float32 elements = 1 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 4 * n, "copies": 5}), flush=True)
 
 
 def bench_aweno(args):

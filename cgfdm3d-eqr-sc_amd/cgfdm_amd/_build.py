@@ -15,7 +15,7 @@ OBJ = os.path.join(ROOT, "_build")
 
 SOURCES = ["setup.hip", "step_fp32.hip", "step_fp64.hip", "step_fp16.hip", "step_bf16.hip", "aweno_fp32.hip",
            "aweno_fp64.hip", "adjoint_fp32.hip", "adjoint_fp64.hip", "capi.cpp"]
-HEADERS = ["common.h", "kernels.h", "setup.h", "cgfdm_step_impl.h", "launch_impl.h", "aweno.h", "aweno_impl.h", "adjoint.h", "adjoint_impl.h"]  # cgfdm_zmarch_disabled.h is not built
+HEADERS = ["common.h", "kernels.h", "setup.h", "cgfdm_step_impl.h", "launch_impl.h", "aweno.h", "aweno_impl.h", "adjoint.h", "adjoint_impl.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -27,8 +27,8 @@
 namespace cg {
 
 // RK weight of h_s in the final state, and of h_{s-1} in the stage-s state
-__device__ __forceinline__ double rk_b(int s) { return s == 0 ? RK_B1 : s == 1 ? RK_B2 : s == 2 ? RK_B3 : RK_B4; }
-__device__ __forceinline__ double rk_a(int s) { return s == 1 ? RK_A2 : s == 2 ? RK_A3 : 1.0; }
+constexpr double rk_b(int s) { return s == 0 ? RK_B1 : s == 1 ? RK_B2 : s == 2 ? RK_B3 : RK_B4; }
+constexpr double rk_a(int s) { return s == 1 ? RK_A2 : s == 2 ? RK_A3 : 1.0; }
 
 template <class R>
 __device__ __forceinline__ void assemble_T(const R* g, R m1, R m2, R m3, R mu, R lam, R b, R* lD) {
@@ -71,16 +71,19 @@ __device__ __forceinline__ void stress_grad(const R* g, const R* V, R m1, R m2, 
                   (m3 * V[0] + m1 * V[2]) * g[4] + (m3 * V[1] + m2 * V[2]) * g[5]);
 }
 
-// adjoint of h at cell p for the current stage: b4 mu in stage 3, G otherwise
-template <class R>
+// adjoint of h at cell p for the current stage: b4 mu in stage 3, G otherwise.
+// The stage is a compile-time parameter of every adjoint kernel, so no kernel
+// chooses base pointers by stage at run time.
+template <class R, int STAGE>
 __device__ __forceinline__ R g_of(const AdjArgs<R>& a, long long q) {
-  return a.stage == 3 ? R(RK_B4) * a.mu[q] : a.G[q];
+  if constexpr (STAGE == 3) return R(RK_B4) * a.mu[q];
+  else return a.G[q];
 }
 
 // ---------------------------------------------------------------------------
 // pass A: per interior cell, products Z, PML auxiliary adjoint, bulk gradient
 // ---------------------------------------------------------------------------
-template <class R, class TM, int FB1, int FB2, int FB3>
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
 __global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restrict__ coef) {
   const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -95,7 +98,7 @@ __global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restri
   R gv[9], gd[9];
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
-    gv[v] = g_of(a, (long long)v * N + p);
+    gv[v] = g_of<R, STAGE>(a, (long long)v * N + p);
     gd[v] = gv[v] * a.DT;
   }
   double gmu = 0, glam = 0, gb = 0;
@@ -152,10 +155,17 @@ __global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restri
 #pragma unroll
           for (int v = 0; v < 9; ++v) {
             const long long e = (long long)v * F.cnt + q;
-            const R gA = a.stage == 3 ? R(RK_B4) * F.mu[e] : F.g[e];
+            R gA, base;
+            if constexpr (STAGE == 3) {
+              gA = R(RK_B4) * F.mu[e];
+              base = F.mu[e];
+            } else {
+              gA = F.g[e];
+              base = F.out[e];
+            }
             const R lA = -ad * a.DT * gA - be * a.DT * gv[v];
-            F.out[e] = (a.stage == 3 ? F.mu[e] : F.out[e]) + lA;
-            if (a.stage > 0) F.g[e] = R(rk_b(a.stage - 1)) * F.mu[e] + R(rk_a(a.stage)) * lA;
+            F.out[e] = base + lA;
+            if constexpr (STAGE > 0) F.g[e] = R(rk_b(STAGE - 1)) * F.mu[e] + R(rk_a(STAGE)) * lA;
             gQ[v] = gA * a.DT;
           }
           R lD[9];
@@ -180,7 +190,7 @@ __global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restri
 // ---------------------------------------------------------------------------
 // pass B: gather of the products, surface accumulator, RK combination
 // ---------------------------------------------------------------------------
-template <class R, int FB1, int FB2, int FB3>
+template <class R, int FB1, int FB2, int FB3, int STAGE>
 __global__ __launch_bounds__(256) void cg_adj_b(AdjArgs<R> a) {
   const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -208,8 +218,9 @@ __global__ __launch_bounds__(256) void cg_adj_b(AdjArgs<R> a) {
       l += a.Ls[q];
       a.Ls[q] = R(0);
     }
-    a.out[q] = (a.stage == 3 ? a.mu[q] : a.out[q]) + l;
-    if (a.stage > 0) a.G[q] = R(rk_b(a.stage - 1)) * a.mu[q] + R(rk_a(a.stage)) * l;
+    if constexpr (STAGE == 3) a.out[q] = a.mu[q] + l;
+    else a.out[q] = a.out[q] + l;
+    if constexpr (STAGE > 0) a.G[q] = R(rk_b(STAGE - 1)) * a.mu[q] + R(rk_a(STAGE)) * l;
   }
 }
 
@@ -233,7 +244,7 @@ __device__ __forceinline__ void scatter_L(R* lf, long long p, long long s, int f
 // d(-DZ^-1 DX), d(-DZ^-1 DY) with respect to mu and lambda of the surface cell,
 // contracted with gRx, gRy (dR = -DZ^-1 (dDZ R + dD)), FP64; chi = 2 lambda + mu
 // as in the forward matrices.
-__device__ inline void rdz_grad_dev(const double* xm, const double* em, const double* zm, double mu, double lam,
+__device__ __forceinline__ void rdz_grad_dev(const double* xm, const double* em, const double* zm, double mu, double lam,
                                     const double* Rx, const double* Ry, const double* gRx, const double* gRy,
                                     double& gmu, double& glam) {
   const double zx = zm[0], zy = zm[1], zz = zm[2];
@@ -302,7 +313,7 @@ __device__ inline void rdz_grad_dev(const double* xm, const double* em, const do
 // surface pass: transposed traction-image rows of the top three layers and the
 // surface term of the x/y PML auxiliaries, scattered into Ls; their gradient
 // ---------------------------------------------------------------------------
-template <class R, class TM, int FB1, int FB2, int FB3>
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
 __global__ __launch_bounds__(256) void cg_adj_s(AdjArgs<R> a, const TM* __restrict__ coef, const R* __restrict__ jac,
                                                 const R* __restrict__ rdzdx, const R* __restrict__ rdzdy) {
   const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
@@ -321,11 +332,10 @@ __global__ __launch_bounds__(256) void cg_adj_s(AdjArgs<R> a, const TM* __restri
   R* Ls = a.Ls;
   R gv[9];
 #pragma unroll
-  for (int v = 0; v < 9; ++v) gv[v] = g_of(a, (long long)v * N + p);
+  for (int v = 0; v < 9; ++v) gv[v] = g_of<R, STAGE>(a, (long long)v * N + p);
   double gmu = 0, glam = 0, gb = 0;
   const R jinv = R(1) / jac[p];
   const R cw[5] = {R(AF_1), R(AF0), R(AF1), R(AF2), R(AF3)};
-  const int FBv[3] = {FB1, FB2, FB3};
 
   // velocity rows: hv = b jinv (d_xi(J xi.T) bx + d_eta(J eta.T) by + d_zeta(J zeta.T)) DT
   auto jt_T = [&](long long q, int c0, const R* lj) {
@@ -471,6 +481,7 @@ __global__ __launch_bounds__(256) void cg_adj_s(AdjArgs<R> a, const TM* __restri
 
   // surface term of the x/y PML auxiliaries on the top layer
   if (k == ktop && P.npml > 0) {
+#pragma unroll
     for (int ax = 0; ax < 2; ++ax) {
       const int q0 = (ax == 0 ? i : j) - H;
       const int n = ax == 0 ? a.nx : a.ny;
@@ -502,7 +513,8 @@ __global__ __launch_bounds__(256) void cg_adj_s(AdjArgs<R> a, const TM* __restri
 #pragma unroll
         for (int v = 0; v < 6; ++v) {
           const long long e = (long long)(3 + v) * F.cnt + q;
-          ga[v] = (a.stage == 3 ? R(RK_B4) * F.mu[e] : F.g[e]) * a.DT;
+          if constexpr (STAGE == 3) ga[v] = R(RK_B4) * F.mu[e] * a.DT;
+          else ga[v] = F.g[e] * a.DT;
         }
         R lVz[3], lV[3] = {0, 0, 0};
         stress_T(ga, cf(M_ZTX, p), cf(M_ZTY, p), cf(M_ZTZ, p), mu, lam, lVz);
@@ -511,7 +523,7 @@ __global__ __launch_bounds__(256) void cg_adj_s(AdjArgs<R> a, const TM* __restri
 #pragma unroll
           for (int c = 0; c < 3; ++c) lV[c] += Rm[(3 * r + c) * SS + sp] * lVz[r];
 #pragma unroll
-        for (int v = 0; v < 3; ++v) scatter_L(Ls + (long long)v * N, p, s, FBv[ax], lV[v] * d, a.rDH);
+        for (int v = 0; v < 3; ++v) scatter_L(Ls + (long long)v * N, p, s, ax == 0 ? FB1 : FB2, lV[v] * d, a.rDH);
         stress_grad(ga, Vz, cf(M_ZTX, p), cf(M_ZTY, p), cf(M_ZTZ, p), gmu, glam);
         double xm[3], em[3], zm[3], Rx[9], Ry[9], gR[9], zero[9];
         for (int c = 0; c < 3; ++c) {
@@ -576,28 +588,44 @@ __global__ void cg_record_vel(const R* __restrict__ W, long long N, int ns, cons
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// the three passes of one adjoint stage for one F/B pattern
+template <class R, class TM, int F1, int F2, int F3, int STAGE>
+void adj_passes(const AdjArgs<R>& a, const TM* coef, const R* jac, const R* rdzdx, const R* rdzdy, dim3 grid, dim3 gs,
+                dim3 block, hipStream_t st) {
+  if (a.fs) cg_adj_s<R, TM, F1, F2, F3, STAGE><<<gs, block, 0, st>>>(a, coef, jac, rdzdx, rdzdy);
+  cg_adj_a<R, TM, F1, F2, F3, STAGE><<<grid, block, 0, st>>>(a, coef);
+  cg_adj_b<R, F1, F2, F3, STAGE><<<grid, block, 0, st>>>(a);
+}
+
+template <class R, class TM, int F1, int F2, int F3>
+void adj_stage_fb(const AdjArgs<R>& a, const TM* coef, const R* jac, const R* rdzdx, const R* rdzdy, dim3 grid, dim3 gs,
+                  dim3 block, hipStream_t st) {
+  switch (a.stage) {
+    case 0: adj_passes<R, TM, F1, F2, F3, 0>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    case 1: adj_passes<R, TM, F1, F2, F3, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    case 2: adj_passes<R, TM, F1, F2, F3, 2>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    default: adj_passes<R, TM, F1, F2, F3, 3>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+  }
+}
+
 template <class R, class TM>
 void launch_adj_stage(AdjArgs<R> a, const TM* coef, const R* jac, const R* rdzdx, const R* rdzdy, int fb1, int fb2,
                       int fb3, hipStream_t st) {
+  if (a.stage < 0 || a.stage > 3) return;
   const dim3 grid((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz), block(256);
   const dim3 gs((a.nx + 63) / 64, (a.ny + 3) / 4, 3);
   // the surface pass reads the auxiliary adjoint g of this stage, which pass A
   // then overwrites with that of the next stage: surface, A, B in this order
-#define CG_ADJ_CASE(F1, F2, F3)                                                        \
-  if (a.fs) cg_adj_s<R, TM, F1, F2, F3><<<gs, block, 0, st>>>(a, coef, jac, rdzdx, rdzdy); \
-  cg_adj_a<R, TM, F1, F2, F3><<<grid, block, 0, st>>>(a, coef);                        \
-  cg_adj_b<R, F1, F2, F3><<<grid, block, 0, st>>>(a);
   switch (fb_index(fb1, fb2, fb3)) {
-    case 0: { CG_ADJ_CASE(-1, -1, -1) } break;
-    case 1: { CG_ADJ_CASE(1, -1, -1) } break;
-    case 2: { CG_ADJ_CASE(-1, 1, -1) } break;
-    case 3: { CG_ADJ_CASE(1, 1, -1) } break;
-    case 4: { CG_ADJ_CASE(-1, -1, 1) } break;
-    case 5: { CG_ADJ_CASE(1, -1, 1) } break;
-    case 6: { CG_ADJ_CASE(-1, 1, 1) } break;
-    default: { CG_ADJ_CASE(1, 1, 1) } break;
+    case 0: adj_stage_fb<R, TM, -1, -1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    case 1: adj_stage_fb<R, TM, 1, -1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    case 2: adj_stage_fb<R, TM, -1, 1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    case 3: adj_stage_fb<R, TM, 1, 1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    case 4: adj_stage_fb<R, TM, -1, -1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    case 5: adj_stage_fb<R, TM, 1, -1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    case 6: adj_stage_fb<R, TM, -1, 1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    default: adj_stage_fb<R, TM, 1, 1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
   }
-#undef CG_ADJ_CASE
 }
 
 template <class R>

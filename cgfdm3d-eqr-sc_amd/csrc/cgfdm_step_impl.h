@@ -66,25 +66,27 @@ __device__ __forceinline__ R mac(const R* f, long long s) {
 
 // CFS-PML auxiliary update for one face at slab index pos (reference
 // pml_deriv.cpp:235-273 and pml_rk.cpp:19-73): hA = (d Q - (alpha+d) A) DT,
-// h -= beta A DT, then the RK4 stage on the auxiliary state.
-template <class R>
+// h -= beta A DT, then the RK4 stage on the auxiliary state. The stage is a
+// compile-time parameter: every kernel instance has one straight-line memory
+// pattern and no stage-dependent choice of base pointers.
+template <class R, int STAGE>
 __device__ __forceinline__ void pml_face(const FaceBuf<R>& F, long long pos, const R (&Q)[9], R d, R ad, R be, R DT,
-                                         int stage, R (&h)[9], bool has_extra, const R (&extra)[6]) {
+                                         R (&h)[9], bool has_extra, const R (&extra)[6]) {
   // extra: surface stress term added to hA[3..8] on the top layer
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
     long long q = (long long)v * F.cnt + pos;
-    R cur = stage == 0 ? F.a[q] : F.b[q];
+    R cur = STAGE == 0 ? F.a[q] : F.b[q];
     R hA = (d * Q[v] - ad * cur) * DT;
     if (has_extra && v >= 3) hA += extra[v - 3] * DT;
     h[v] -= be * cur * DT;
-    if (stage == 0) {
+    if constexpr (STAGE == 0) {
       F.t[q] = cur + R(RK_B1) * hA;
       F.b[q] = cur + R(RK_A2) * hA;
-    } else if (stage == 1) {
+    } else if constexpr (STAGE == 1) {
       F.t[q] = F.t[q] + R(RK_B2) * hA;
       F.b[q] = F.a[q] + R(RK_A3) * hA;
-    } else if (stage == 2) {
+    } else if constexpr (STAGE == 2) {
       F.t[q] = F.t[q] + R(RK_B3) * hA;
       F.b[q] = F.a[q] + hA;
     } else {
@@ -98,7 +100,7 @@ __device__ __forceinline__ void pml_face(const FaceBuf<R>& F, long long pos, con
 // the unscaled zeta difference of field v and `wc(v)` the centre value, so the
 // same body serves the thread-per-cell and the z-marching kernels.
 // ---------------------------------------------------------------------------
-template <class R, class TM, int FB1, int FB2, int FB3, class ZD, class WC>
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE, class ZD, class WC>
 __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __restrict__ coef, int i, int j, int k,
                                           ZD zd, WC wc) {
   const long long N = a.N, S = a.S;
@@ -125,8 +127,8 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
     if (x1 || x2) {
       const R d = P.d[0][i], ad = d + P.alpha[0][i];
       const long long row = (long long)(j - H) + (long long)a.ny * (k - H);
-      if (x1) pml_face<R>(P.f[0], (long long)(i - H) + (long long)P.npml * row, Q, d, ad, bx, a.DT, a.stage, h, false, none);
-      if (x2) pml_face<R>(P.f[3], (long long)(i - H - (a.nx - P.npml)) + (long long)P.npml * row, Q, d, ad, bx, a.DT, a.stage, h, false, none);
+      if (x1) pml_face<R, STAGE>(P.f[0], (long long)(i - H) + (long long)P.npml * row, Q, d, ad, bx, a.DT, h, false, none);
+      if (x2) pml_face<R, STAGE>(P.f[3], (long long)(i - H - (a.nx - P.npml)) + (long long)P.npml * row, Q, d, ad, bx, a.DT, h, false, none);
     }
   }
   // y: derivatives along eta (wave_deriv.cpp:224-232)
@@ -144,8 +146,8 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
     if (y1 || y2) {
       const R d = P.d[1][j], ad = d + P.alpha[1][j];
       const long long i0 = i - H, k0 = k - H;
-      if (y1) pml_face<R>(P.f[1], i0 + (long long)a.nx * ((j - H) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, false, none);
-      if (y2) pml_face<R>(P.f[4], i0 + (long long)a.nx * ((j - H - (a.ny - P.npml)) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, false, none);
+      if (y1) pml_face<R, STAGE>(P.f[1], i0 + (long long)a.nx * ((j - H) + (long long)P.npml * k0), Q, d, ad, by, a.DT, h, false, none);
+      if (y2) pml_face<R, STAGE>(P.f[4], i0 + (long long)a.nx * ((j - H - (a.ny - P.npml)) + (long long)P.npml * k0), Q, d, ad, by, a.DT, h, false, none);
     }
   }
   // z: derivatives along zeta (wave_deriv.cpp:234-242)
@@ -163,22 +165,22 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
     if (z1 || z2) {
       const R d = P.d[2][k], ad = d + P.alpha[2][k];
       const long long i0 = i - H, j0 = j - H;
-      if (z1) pml_face<R>(P.f[2], i0 + (long long)a.nx * (j0 + (long long)a.ny * (k - H)), Q, d, ad, bz, a.DT, a.stage, h, false, none);
-      if (z2) pml_face<R>(P.f[5], i0 + (long long)a.nx * (j0 + (long long)a.ny * (k - H - (a.nz - P.npml))), Q, d, ad, bz, a.DT, a.stage, h, false, none);
+      if (z1) pml_face<R, STAGE>(P.f[2], i0 + (long long)a.nx * (j0 + (long long)a.ny * (k - H)), Q, d, ad, bz, a.DT, h, false, none);
+      if (z2) pml_face<R, STAGE>(P.f[5], i0 + (long long)a.nx * (j0 + (long long)a.ny * (k - H - (a.nz - P.npml))), Q, d, ad, bz, a.DT, h, false, none);
     }
   }
   // RK4 stage update of W (wave_rk.cpp:18-104)
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
     const long long q = (long long)v * N + p;
-    if (a.stage == 0) {
+    if constexpr (STAGE == 0) {
       const R m = wc(v);
       a.wt[q] = m + R(RK_B1) * h[v];
       a.wout[q] = m + R(RK_A2) * h[v];
-    } else if (a.stage == 1) {
+    } else if constexpr (STAGE == 1) {
       a.wt[q] = a.wt[q] + R(RK_B2) * h[v];
       a.wout[q] = a.wm[q] + R(RK_A3) * h[v];
-    } else if (a.stage == 2) {
+    } else if constexpr (STAGE == 2) {
       a.wt[q] = a.wt[q] + R(RK_B3) * h[v];
       a.wout[q] = a.wm[q] + h[v];
     } else {
@@ -190,7 +192,7 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
 // ---------------------------------------------------------------------------
 // interior kernel: one thread per cell below the surface layers
 // ---------------------------------------------------------------------------
-template <class R, class TM, int FB1, int FB2, int FB3>
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
 __global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restrict__ coef) {
   const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -199,7 +201,7 @@ __global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restr
   const long long N = a.N, S = a.S;
   const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
   const R* W = a.win;
-  cell_body<R, TM, FB1, FB2, FB3>(
+  cell_body<R, TM, FB1, FB2, FB3, STAGE>(
       a, coef, i, j, k, [&](int v) -> R { return mac<FB3>(W + (long long)v * N + p, S); },
       [&](int v) -> R { return W[(long long)v * N + p]; });
 }
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restr
 // ---------------------------------------------------------------------------
 // surface kernel: top three layers, traction image
 // ---------------------------------------------------------------------------
-template <class R, class TM, int FB1, int FB2, int FB3>
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
 __global__ __launch_bounds__(256) void cg_surface(StageArgs<R> a, const TM* __restrict__ coef, SurfArgs<R> s) {
   const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -322,8 +324,8 @@ __global__ __launch_bounds__(256) void cg_surface(StageArgs<R> a, const TM* __re
       stress_rows(Vz, cf(M_ZTX, p), cf(M_ZTY, p), cf(M_ZTZ, p), mu, lam, ex);
     }
     long long rowq = (long long)(j - H) + (long long)a.ny * (k - H);
-    if (in_x1) pml_face<R>(P.f[0], (long long)(i - H) + (long long)P.npml * rowq, Q, d, ad, bx, a.DT, a.stage, h, top, ex);
-    if (in_x2) pml_face<R>(P.f[3], (long long)(i - H - (a.nx - P.npml)) + (long long)P.npml * rowq, Q, d, ad, bx, a.DT, a.stage, h, top, ex);
+    if (in_x1) pml_face<R, STAGE>(P.f[0], (long long)(i - H) + (long long)P.npml * rowq, Q, d, ad, bx, a.DT, h, top, ex);
+    if (in_x2) pml_face<R, STAGE>(P.f[3], (long long)(i - H - (a.nx - P.npml)) + (long long)P.npml * rowq, Q, d, ad, bx, a.DT, h, top, ex);
   }
   if (in_y1 || in_y2) {
     R Q[9];
@@ -339,8 +341,8 @@ __global__ __launch_bounds__(256) void cg_surface(StageArgs<R> a, const TM* __re
       stress_rows(Vz, cf(M_ZTX, p), cf(M_ZTY, p), cf(M_ZTZ, p), mu, lam, ex);
     }
     long long i0 = i - H, k0 = k - H;
-    if (in_y1) pml_face<R>(P.f[1], i0 + (long long)a.nx * ((j - H) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, top, ex);
-    if (in_y2) pml_face<R>(P.f[4], i0 + (long long)a.nx * ((j - H - (a.ny - P.npml)) + (long long)P.npml * k0), Q, d, ad, by, a.DT, a.stage, h, top, ex);
+    if (in_y1) pml_face<R, STAGE>(P.f[1], i0 + (long long)a.nx * ((j - H) + (long long)P.npml * k0), Q, d, ad, by, a.DT, h, top, ex);
+    if (in_y2) pml_face<R, STAGE>(P.f[4], i0 + (long long)a.nx * ((j - H - (a.ny - P.npml)) + (long long)P.npml * k0), Q, d, ad, by, a.DT, h, top, ex);
   }
   if (in_z1) {  // only for very shallow grids: bottom PML reaching the surface layers
     R Dz[9], Q[9];
@@ -349,20 +351,20 @@ __global__ __launch_bounds__(256) void cg_surface(StageArgs<R> a, const TM* __re
     assemble(Dz, cf(M_ZTX, p), cf(M_ZTY, p), cf(M_ZTZ, p), mu, lam, bu, Q);
     R d = P.d[2][k], ad = d + P.alpha[2][k], bz = P.beta[2][k];
     const R none[6] = {0, 0, 0, 0, 0, 0};
-    pml_face<R>(P.f[2], (long long)(i - H) + (long long)a.nx * ((j - H) + (long long)a.ny * (k - H)), Q, d, ad, bz, a.DT, a.stage, h, false, none);
+    pml_face<R, STAGE>(P.f[2], (long long)(i - H) + (long long)a.nx * ((j - H) + (long long)a.ny * (k - H)), Q, d, ad, bz, a.DT, h, false, none);
   }
   R out[9];
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
     long long q = (long long)v * N + p;
-    if (a.stage == 0) {
+    if constexpr (STAGE == 0) {
       R m = W[q];
       a.wt[q] = m + R(RK_B1) * h[v];
       out[v] = m + R(RK_A2) * h[v];
-    } else if (a.stage == 1) {
+    } else if constexpr (STAGE == 1) {
       a.wt[q] = a.wt[q] + R(RK_B2) * h[v];
       out[v] = a.wm[q] + R(RK_A3) * h[v];
-    } else if (a.stage == 2) {
+    } else if constexpr (STAGE == 2) {
       a.wt[q] = a.wt[q] + R(RK_B3) * h[v];
       out[v] = a.wm[q] + h[v];
     } else {
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(256) void cg_surface(StageArgs<R> a, const TM* __re
     }
     a.wout[q] = out[v];
   }
-  if (a.stage == 3 && k == ktop && s.pgv != nullptr) {
+  if (STAGE == 3 && k == ktop && s.pgv != nullptr) {
     // observers on the final state of the step (PGV.cpp:62-175, solve_displacement.cpp:28-63)
     float fx = (float)out[0], fy = (float)out[1], fz = (float)out[2];
     float vh = sqrtf(fx * fx + fy * fy), va = sqrtf(fx * fx + fy * fy + fz * fz);

@@ -1,4 +1,4 @@
-// Host launchers of the CGFDM step kernels: one direct launch per F/B pattern.
+// Host launchers of the CGFDM step kernels: one direct launch per F/B pattern and RK stage.
 #pragma once
 #include "cgfdm_step_impl.h"
 
@@ -7,35 +7,58 @@ namespace cg {
 // F/B pattern index: bit 0 = FB1 > 0, bit 1 = FB2 > 0, bit 2 = FB3 > 0
 inline int fb_index(int fb1, int fb2, int fb3) { return (fb1 > 0) + 2 * (fb2 > 0) + 4 * (fb3 > 0); }
 
+// one kernel instance per F/B pattern and RK stage (the stage is a template
+// parameter of the kernels: no stage-dependent control flow on the device)
+template <class R, class TM, int FB1, int FB2, int FB3>
+void cell_stage(const StageArgs<R>& a, const TM* coef, dim3 g, dim3 b, hipStream_t st) {
+  switch (a.stage) {
+    case 0: cg_cell<R, TM, FB1, FB2, FB3, 0><<<g, b, 0, st>>>(a, coef); break;
+    case 1: cg_cell<R, TM, FB1, FB2, FB3, 1><<<g, b, 0, st>>>(a, coef); break;
+    case 2: cg_cell<R, TM, FB1, FB2, FB3, 2><<<g, b, 0, st>>>(a, coef); break;
+    default: cg_cell<R, TM, FB1, FB2, FB3, 3><<<g, b, 0, st>>>(a, coef); break;
+  }
+}
+
+template <class R, class TM, int FB1, int FB2, int FB3>
+void surface_stage(const StageArgs<R>& a, const TM* coef, const SurfArgs<R>& s, dim3 g, dim3 b, hipStream_t st) {
+  switch (a.stage) {
+    case 0: cg_surface<R, TM, FB1, FB2, FB3, 0><<<g, b, 0, st>>>(a, coef, s); break;
+    case 1: cg_surface<R, TM, FB1, FB2, FB3, 1><<<g, b, 0, st>>>(a, coef, s); break;
+    case 2: cg_surface<R, TM, FB1, FB2, FB3, 2><<<g, b, 0, st>>>(a, coef, s); break;
+    default: cg_surface<R, TM, FB1, FB2, FB3, 3><<<g, b, 0, st>>>(a, coef, s); break;
+  }
+}
+
 template <class R, class TM>
 void launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, hipStream_t st) {
   const int nk = a.kend - a.kbeg;
-  if (nk <= 0 || a.nx <= 0 || a.ny <= 0) return;
+  if (nk <= 0 || a.nx <= 0 || a.ny <= 0 || a.stage < 0 || a.stage > 3) return;
   const dim3 grid((a.nx + 63) / 64, (a.ny + 3) / 4, nk), block(256);
   switch (fb_index(fb1, fb2, fb3)) {
-    case 0: cg_cell<R, TM, -1, -1, -1><<<grid, block, 0, st>>>(a, coef); break;
-    case 1: cg_cell<R, TM, 1, -1, -1><<<grid, block, 0, st>>>(a, coef); break;
-    case 2: cg_cell<R, TM, -1, 1, -1><<<grid, block, 0, st>>>(a, coef); break;
-    case 3: cg_cell<R, TM, 1, 1, -1><<<grid, block, 0, st>>>(a, coef); break;
-    case 4: cg_cell<R, TM, -1, -1, 1><<<grid, block, 0, st>>>(a, coef); break;
-    case 5: cg_cell<R, TM, 1, -1, 1><<<grid, block, 0, st>>>(a, coef); break;
-    case 6: cg_cell<R, TM, -1, 1, 1><<<grid, block, 0, st>>>(a, coef); break;
-    default: cg_cell<R, TM, 1, 1, 1><<<grid, block, 0, st>>>(a, coef); break;
+    case 0: cell_stage<R, TM, -1, -1, -1>(a, coef, grid, block, st); break;
+    case 1: cell_stage<R, TM, 1, -1, -1>(a, coef, grid, block, st); break;
+    case 2: cell_stage<R, TM, -1, 1, -1>(a, coef, grid, block, st); break;
+    case 3: cell_stage<R, TM, 1, 1, -1>(a, coef, grid, block, st); break;
+    case 4: cell_stage<R, TM, -1, -1, 1>(a, coef, grid, block, st); break;
+    case 5: cell_stage<R, TM, 1, -1, 1>(a, coef, grid, block, st); break;
+    case 6: cell_stage<R, TM, -1, 1, 1>(a, coef, grid, block, st); break;
+    default: cell_stage<R, TM, 1, 1, 1>(a, coef, grid, block, st); break;
   }
 }
 
 template <class R, class TM>
 void launch_surface(StageArgs<R> a, const TM* coef, SurfArgs<R> s, int fb1, int fb2, int fb3, hipStream_t st) {
+  if (a.stage < 0 || a.stage > 3) return;
   const dim3 grid((a.nx + 63) / 64, (a.ny + 3) / 4, 3), block(256);
   switch (fb_index(fb1, fb2, fb3)) {
-    case 0: cg_surface<R, TM, -1, -1, -1><<<grid, block, 0, st>>>(a, coef, s); break;
-    case 1: cg_surface<R, TM, 1, -1, -1><<<grid, block, 0, st>>>(a, coef, s); break;
-    case 2: cg_surface<R, TM, -1, 1, -1><<<grid, block, 0, st>>>(a, coef, s); break;
-    case 3: cg_surface<R, TM, 1, 1, -1><<<grid, block, 0, st>>>(a, coef, s); break;
-    case 4: cg_surface<R, TM, -1, -1, 1><<<grid, block, 0, st>>>(a, coef, s); break;
-    case 5: cg_surface<R, TM, 1, -1, 1><<<grid, block, 0, st>>>(a, coef, s); break;
-    case 6: cg_surface<R, TM, -1, 1, 1><<<grid, block, 0, st>>>(a, coef, s); break;
-    default: cg_surface<R, TM, 1, 1, 1><<<grid, block, 0, st>>>(a, coef, s); break;
+    case 0: surface_stage<R, TM, -1, -1, -1>(a, coef, s, grid, block, st); break;
+    case 1: surface_stage<R, TM, 1, -1, -1>(a, coef, s, grid, block, st); break;
+    case 2: surface_stage<R, TM, -1, 1, -1>(a, coef, s, grid, block, st); break;
+    case 3: surface_stage<R, TM, 1, 1, -1>(a, coef, s, grid, block, st); break;
+    case 4: surface_stage<R, TM, -1, -1, 1>(a, coef, s, grid, block, st); break;
+    case 5: surface_stage<R, TM, 1, -1, 1>(a, coef, s, grid, block, st); break;
+    case 6: surface_stage<R, TM, -1, 1, 1>(a, coef, s, grid, block, st); break;
+    default: surface_stage<R, TM, 1, 1, 1>(a, coef, s, grid, block, st); break;
   }
 }
 

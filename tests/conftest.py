@@ -3,13 +3,12 @@ import sys
 
 import pytest
 
-# The CGFDM step kernels are switched off in the library (the step returns
-# CGFDM_ERR_DISABLED): their first MI355X run, the FP64 24x20x16 PML + free
-# surface case, ended in a GPU memory fault whose cause has not been shown.
-# Every GPU test that steps the CGFDM solver carries this mark.
-CGFDM_STEP_OFF = pytest.mark.skip(
-    reason="CGFDM step kernels switched off: unexplained GPU memory fault on their first run "
-           "(test_fp64_wavefield_matches_oracle[curvilinear_pml_fs])")
+# The CGFDM adjoint (forward_record + adjoint) is switched off in the library
+# (CGFDM_ERR_DISABLED) until its kernels, rebuilt with the RK stage as a template
+# parameter, have been executed from their compiled code with every access
+# checked. The GPU tests that call it carry this mark.
+CGFDM_ADJOINT_OFF = pytest.mark.skip(
+    reason="CGFDM adjoint switched off until its rebuilt kernels are verified from the compiled code")
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "cgfdm3d-eqr-sc_amd"))

@@ -11,9 +11,9 @@ import pytest
 
 from cgfdm_amd import model as M
 
-from conftest import CGFDM_STEP_OFF
+from conftest import CGFDM_ADJOINT_OFF
 
-pytestmark = [pytest.mark.gpu, CGFDM_STEP_OFF]
+pytestmark = [pytest.mark.gpu, CGFDM_ADJOINT_OFF]
 
 
 @pytest.fixture(scope="module")

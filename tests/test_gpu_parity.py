@@ -8,7 +8,6 @@ import numpy as np
 import pytest
 
 from cgfdm_amd import model as M
-from conftest import CGFDM_STEP_OFF
 
 pytestmark = pytest.mark.gpu
 H = M.HALO
@@ -53,7 +52,6 @@ def make(name, nt=12):
     return m
 
 
-@CGFDM_STEP_OFF
 @pytest.mark.parametrize("name", list(CASES))
 def test_fp64_wavefield_matches_oracle(name, oracle_mod, solver_mod):
     m = make(name)
@@ -64,7 +62,6 @@ def test_fp64_wavefield_matches_oracle(name, oracle_mod, solver_mod):
     np.testing.assert_allclose(g.stations(), o.stations(), rtol=1e-9, atol=1e-30)
 
 
-@CGFDM_STEP_OFF
 @pytest.mark.parametrize("name", list(CASES))
 def test_fp32_wavefield_matches_oracle(name, oracle_mod, solver_mod):
     m = make(name)
@@ -86,7 +83,6 @@ def test_setup_products_match_oracle(oracle_mod, solver_mod):
         assert rg[k] == pytest.approx(ro[k], rel=1e-12), k
 
 
-@CGFDM_STEP_OFF
 def test_fp32_observers_match_fp64_oracle(oracle_mod, solver_mod):
     """Station seismograms, PGV and surface displacement of the FP32 GPU run
     against the FP64 oracle (relative L2 <= 1e-4)."""
@@ -101,7 +97,6 @@ def test_fp32_observers_match_fp64_oracle(oracle_mod, solver_mod):
     assert rel_l2(g.surface_displacement(), o64.surface_dis()[:, H:-H, H:-H]) < 1e-4
 
 
-@CGFDM_STEP_OFF
 @pytest.mark.parametrize("coef,tol", [(1, 5e-3), (2, 4e-2)])
 def test_16bit_coefficients_vs_fp64_oracle(coef, tol, oracle_mod, solver_mod):
     """FP32 wavefield with FP16 (coef=1) or BF16 (coef=2) storage of the metric and
@@ -116,13 +111,13 @@ def test_16bit_coefficients_vs_fp64_oracle(coef, tol, oracle_mod, solver_mod):
     assert rel_l2(g.pgv(), o64.pgv()) < tol
 
 
-def test_cgfdm_step_is_switched_off(solver_mod):
-    """Stepping the CGFDM solver fails loudly (no kernel is launched) while its
-    step kernels are switched off; the AWENO scheme is unaffected."""
+def test_adjoint_is_switched_off(solver_mod):
+    """forward_record fails loudly (no kernel is launched) while the adjoint is
+    switched off, and leaves the solver where it was."""
     m = make("no_pml_flat")
-    g = solver_mod.Solver(m, precision=4)
+    g = solver_mod.Solver(m, precision=8)
     with pytest.raises(solver_mod.CgfdmError) as e:
-        g.step(1)
+        g.forward_record(2, 1)
     assert e.value.code == 4
     assert g.it == 0
 
