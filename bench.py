@@ -89,7 +89,8 @@ def main():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--coef", type=int, default=0, help="0 fp32, 1 fp16, 2 bf16 coefficient storage")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="steps of the CPU baseline sample (default 30 CGFDM, 10 ME-AWENO: about 10 s on 16 threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--scheme", choices=["cgfdm", "aweno"], default="cgfdm")
@@ -110,6 +111,8 @@ def main():
     from cgfdm_amd import model as M
     from cgfdm_amd import solver as S
 
+    if args.cpu_steps is None:
+        args.cpu_steps = 10 if args.scheme == "aweno" else 30
     if args.copy_calibration:
         return copy_calibration()
     if args.scheme == "aweno":

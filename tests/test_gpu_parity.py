@@ -59,7 +59,12 @@ def test_fp64_wavefield_matches_oracle(name, oracle_mod, solver_mod):
     wg, wo = g.wavefield(), o.wavefield()
     for v in range(9):
         assert rel_l2(wg[v], wo[v]) < 1e-10, (name, v)
-    np.testing.assert_allclose(g.stations(), o.stations(), rtol=1e-9, atol=1e-30)
+    # stations: velocity and stress families each to 1e-9 relative, with an
+    # absolute floor of 1e-12 of the family's peak for entries that are zero
+    # in the oracle up to rounding (traction-free components on a flat surface)
+    sg, so = g.stations(), o.stations()
+    for fam in (slice(0, 3), slice(3, 9)):
+        np.testing.assert_allclose(sg[..., fam], so[..., fam], rtol=1e-9, atol=1e-12 * np.abs(so[..., fam]).max())
 
 
 @pytest.mark.parametrize("name", list(CASES))
