@@ -158,7 +158,7 @@ def main():
     achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
     step_bytes = 4 * stage_bytes(args.coef) * cells
     hbm_step = step_bytes / (ms_step * 1e-3) / 1e9
-    traffic = traffic_from_profiles("traffic_interior_config2.json")
+    traffic = traffic_from_profiles("cg_cell_512x512x256_traffic.json")
 
     # accuracy leg: FP32 vs FP64 GPU on the 128 x 128 x 64 half-space (config 1), stations
     l2 = None
