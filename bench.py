@@ -275,10 +275,16 @@ def copy_calibration():
     a = torch.ones(n, dtype=torch.float32, device="cuda:0")
     b = torch.empty_like(a)
     torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(5):
         b.copy_(a)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": 4 * n, "copies": 5}), flush=True)
+    ms = e0.elapsed_time(e1) / 5
+    print(json.dumps({"copy_bytes": 4 * n, "copies": 5, "ms_per_copy": ms,
+                      "read_plus_write_GBps": 2 * 4 * n / (ms * 1e-3) / 1e9}), flush=True)
 
 
 def bench_aweno(args):

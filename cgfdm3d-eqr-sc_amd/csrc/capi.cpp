@@ -215,8 +215,25 @@ struct Solver : SolverBase {
   R* adj_src = nullptr;
   R* adj_end = nullptr;
 
+  // interior block order: regions of 64 x (4 rty) columns per XCD; the
+  // environment variable CGFDM_INTERIOR_RTY (1..64) overrides it for tuning
+  int rty = 4;
+  // z-marching interior kernel (FP32 wavefield) with chunks of kc planes; 0
+  // selects the thread-per-cell kernel. CGFDM_INTERIOR_KC (0..4096) overrides it.
+  int kc = 0;
+  int zty = 8;  // z-marching tile height (CGFDM_INTERIOR_ZTY: 4 or 8)
+
   int init(const cgfdm_desc& dd, const float* coord, const float* medium) override {
     d = dd;
+    if (const char* e = std::getenv("CGFDM_INTERIOR_RTY")) {
+      const int v = std::atoi(e);
+      if (v >= 1 && v <= 64) rty = v;
+    }
+    if (const char* e = std::getenv("CGFDM_INTERIOR_KC")) {
+      const int v = std::atoi(e);
+      if (v >= 0 && v <= 4096) kc = v;
+    }
+    if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
     NX = d.nx + 2 * cg::H;
     NY = d.ny + 2 * cg::H;
     NZ = d.nz + 2 * cg::H;
@@ -451,6 +468,9 @@ struct Solver : SolverBase {
         a.kbeg = cg::H;
         a.kend = kend;
         a.stage = irk;
+        a.rty = rty;
+        a.kc = kc;
+        a.zty = zty;
         a.DT = R(d.dt);
         a.rDH = R(1.0 / d.dh);
         a.win = irk == 0 ? A : irk == 1 ? B : irk == 2 ? C : B;
@@ -624,7 +644,7 @@ struct Solver : SolverBase {
     for (int irk = 0; irk < 3; ++irk) {
       cg::StageArgs<R> a{};
       a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.NX = NX; a.NY = NY; a.N = N; a.S = S;
-      a.kbeg = cg::H; a.kend = kend; a.stage = irk;
+      a.kbeg = cg::H; a.kend = kend; a.stage = irk; a.rty = rty; a.kc = kc; a.zty = zty;
       a.DT = R(d.dt); a.rDH = R(1.0 / d.dh);
       a.win = ins[irk]; a.wout = outs[irk]; a.wm = A; a.wt = T; a.pml = pml;
       cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream);
@@ -804,8 +824,25 @@ struct AwenoSolver : SolverBase {
   R* src_rate = nullptr;
   std::vector<double> jac_host;  // FP64 J for the source scaling
 
+  // interior block order: regions of 64 x (4 rty) columns per XCD; the
+  // environment variable CGFDM_INTERIOR_RTY (1..64) overrides it for tuning
+  int rty = 4;
+  // z-marching interior kernel (FP32 wavefield) with chunks of kc planes; 0
+  // selects the thread-per-cell kernel. CGFDM_INTERIOR_KC (0..4096) overrides it.
+  int kc = 0;
+  int zty = 8;  // z-marching tile height (CGFDM_INTERIOR_ZTY: 4 or 8)
+
   int init(const cgfdm_desc& dd, const float* coord, const float* medium) override {
     d = dd;
+    if (const char* e = std::getenv("CGFDM_INTERIOR_RTY")) {
+      const int v = std::atoi(e);
+      if (v >= 1 && v <= 64) rty = v;
+    }
+    if (const char* e = std::getenv("CGFDM_INTERIOR_KC")) {
+      const int v = std::atoi(e);
+      if (v >= 0 && v <= 4096) kc = v;
+    }
+    if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
     NX = d.nx + 2 * cg::H;
     NY = d.ny + 2 * cg::H;
     NZ = d.nz + 2 * cg::H;

@@ -96,16 +96,17 @@ __device__ __forceinline__ void pml_face(const FaceBuf<R>& F, long long pos, con
 }
 
 // ---------------------------------------------------------------------------
-// interior cell update: 27 derivatives, PML faces, RK4 stage. `zd(v)` returns
-// the unscaled zeta difference of field v and `wc(v)` the centre value, so the
-// same body serves the thread-per-cell and the z-marching kernels.
+// interior cell update: 27 derivatives, PML faces, RK4 stage. `xd(v)`, `yd(v)`
+// and `zd(v)` return the unscaled xi, eta and zeta differences of field v and
+// `wc(v)` the centre value, so the same body serves the thread-per-cell kernel
+// (differences from global memory) and the z-marching kernel (x/y from an LDS
+// plane, z from a register window).
 // ---------------------------------------------------------------------------
-template <class R, class TM, int FB1, int FB2, int FB3, int STAGE, class ZD, class WC>
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE, class XD, class YD, class ZD, class WC>
 __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __restrict__ coef, int i, int j, int k,
-                                          ZD zd, WC wc) {
+                                          XD xd, YD yd, ZD zd, WC wc) {
   const long long N = a.N, S = a.S;
   const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
-  const R* W = a.win;
   auto cf = [&](int c) -> R { return R(Store<TM>::ld(coef + (long long)c * N + p)); };
   const R mu = cf(M_MU), lam = cf(M_LAM), bu = cf(M_B);
   const PmlArgs<R>& P = a.pml;
@@ -113,7 +114,7 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
 
   // x: derivatives along xi (wave_deriv.cpp:214-222), scaled by 1/beta_x
 #pragma unroll
-  for (int v = 0; v < 9; ++v) D[v] = mac<FB1>(W + (long long)v * N + p, 1);
+  for (int v = 0; v < 9; ++v) D[v] = xd(v);
   assemble(D, cf(M_XIX), cf(M_XIY), cf(M_XIZ), mu, lam, bu, Q);
   const R bx = P.beta[0][i];
 #pragma unroll
@@ -133,7 +134,7 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
   }
   // y: derivatives along eta (wave_deriv.cpp:224-232)
 #pragma unroll
-  for (int v = 0; v < 9; ++v) D[v] = mac<FB2>(W + (long long)v * N + p, a.NX);
+  for (int v = 0; v < 9; ++v) D[v] = yd(v);
   assemble(D, cf(M_ETX), cf(M_ETY), cf(M_ETZ), mu, lam, bu, Q);
   const R by = P.beta[1][j];
 #pragma unroll
@@ -192,18 +193,196 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
 // ---------------------------------------------------------------------------
 // interior kernel: one thread per cell below the surface layers
 // ---------------------------------------------------------------------------
+
+// XCD-aware block order of the interior launch. The hardware hands block b to
+// XCD b % 8, so block b is the (b / 8)-th block that XCD runs. The (x, y) plane
+// is cut into regions of one 64-column tile by rty 4-row tiles; regions are dealt
+// to the XCDs in turn and each XCD walks its regions one after another, plane by
+// plane in k and tile by tile within a plane. The z and y stencil neighbours of a
+// cell are then read by blocks of the same XCD a few planes earlier or later, and
+// come from that XCD's L2 instead of HBM. Returns false for padding blocks.
+struct CellTile {
+  int i0, j0, k;
+};
+__device__ __forceinline__ bool interior_tile(int b, int nx, int ny, int nk, int rty, CellTile& t) {
+  const int ntx = (nx + 63) / 64, nty = (ny + 3) / 4;
+  const int nry = (nty + rty - 1) / rty;
+  const int per_region = rty * nk;
+  const int xcd = b & 7, seq = b >> 3;
+  const int slot = seq / per_region, within = seq - slot * per_region;
+  const int r = slot * 8 + xcd;
+  const int kk = within / rty, ty = within - kk * rty;
+  const int rx = r % ntx, ry = r / ntx;
+  const int tyg = ry * rty + ty;
+  t.i0 = rx * 64;
+  t.j0 = tyg * 4;
+  t.k = kk;
+  return r < ntx * nry && tyg < nty;
+}
+
+// number of blocks of the interior launch for interior_tile()
+inline long long interior_blocks(int nx, int ny, int nk, int rty) {
+  const long long ntx = (nx + 63) / 64, nty = (ny + 3) / 4;
+  const long long nregions = ntx * ((nty + rty - 1) / rty);
+  return 8 * ((nregions + 7) / 8) * (long long)rty * nk;
+}
+
 template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
 __global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restrict__ coef) {
-  const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int k = a.kbeg + blockIdx.z;
-  if (i >= H + a.nx || j >= H + a.ny || k >= a.kend) return;
+  CellTile t;
+  if (!interior_tile(blockIdx.x, a.nx, a.ny, a.kend - a.kbeg, a.rty, t)) return;
+  const int i = H + t.i0 + (threadIdx.x & 63);
+  const int j = H + t.j0 + (threadIdx.x >> 6);
+  const int k = a.kbeg + t.k;
+  if (i >= H + a.nx || j >= H + a.ny) return;
   const long long N = a.N, S = a.S;
   const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
   const R* W = a.win;
+  const long long NXl = a.NX;
   cell_body<R, TM, FB1, FB2, FB3, STAGE>(
-      a, coef, i, j, k, [&](int v) -> R { return mac<FB3>(W + (long long)v * N + p, S); },
+      a, coef, i, j, k, [&](int v) -> R { return mac<FB1>(W + (long long)v * N + p, 1); },
+      [&](int v) -> R { return mac<FB2>(W + (long long)v * N + p, NXl); },
+      [&](int v) -> R { return mac<FB3>(W + (long long)v * N + p, S); },
       [&](int v) -> R { return W[(long long)v * N + p]; });
+}
+
+// ---------------------------------------------------------------------------
+// z-marching interior kernel: a block of 64 x TY threads owns 64 x TY columns
+// and walks a chunk of kc planes up z. Per plane it reads the nine fields once:
+// the block's own cells into a per-thread register window over z (planes
+// k - ZL .. k - ZL + 4, the 5-point zeta stencil of this F/B pattern), the x/y
+// halo (1 cell on one side, 3 on the other, per the F/B signs) from global
+// memory into a double-buffered LDS plane, which also receives the window's
+// centre values. The next plane's window entry and halo are prefetched into
+// registers while the current plane is computed. Columns past the interior
+// (tile overhang) load nothing and hold zeros, which is what the ghost cells
+// they stand for contain.
+// ---------------------------------------------------------------------------
+template <int FB1, int FB2, int TY>
+struct ZTile {
+  static constexpr int TX = 64;
+  static constexpr int XL = FB1 > 0 ? 1 : 3, XR = 4 - XL;  // x halo left / right
+  static constexpr int YL = FB2 > 0 ? 1 : 3, YR = 4 - YL;  // y halo below / above
+  static constexpr int LX = TX + 4, LY = TY + 4, LP = LX * LY;
+  static constexpr int NT = TX * TY;
+  static constexpr int NHALO = 4 * LX + TY * 4;             // halo cells per field
+  static constexpr int NHR = (9 * NHALO + NT - 1) / NT;      // halo loads per thread
+  // tile-local LDS coordinates (lx, ly) of halo entry q of one field
+  static __device__ __forceinline__ void halo_xy(int q, int& lx, int& ly) {
+    if (q < YL * LX) {
+      ly = q / LX;
+      lx = q - ly * LX;
+    } else if (q < YL * LX + TY * 4) {
+      const int r = q - YL * LX, c = r & 3;
+      ly = YL + (r >> 2);
+      lx = c < XL ? c : TX + c;
+    } else {
+      const int r = q - YL * LX - TY * 4;
+      ly = YL + TY + r / LX;
+      lx = r - (ly - YL - TY) * LX;
+    }
+  }
+};
+
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE, int TY>
+__global__ __launch_bounds__(64 * TY) void cg_zcol(StageArgs<R> a, const TM* __restrict__ coef) {
+  using T = ZTile<FB1, FB2, TY>;
+  constexpr int ZL = FB3 > 0 ? 1 : 3;  // window slot m holds plane k - ZL + m
+  __shared__ R sw[2][9][T::LP];
+  const int ntx = (a.nx + 63) / 64, nty = (a.ny + TY - 1) / TY;
+  const int b = blockIdx.x;
+  const int tile_x = b % ntx, tile_y = (b / ntx) % nty, chunk = b / (ntx * nty);
+  const int k0 = a.kbeg + chunk * a.kc;
+  const int k1 = min(k0 + a.kc, a.kend);
+  if (k0 >= k1) return;  // uniform per block, before any barrier
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int i = H + tile_x * 64 + tx, j = H + tile_y * TY + ty;
+  const bool active = i < H + a.nx && j < H + a.ny;
+  const long long N = a.N, S = a.S, NXl = a.NX;
+  const long long col = (long long)i + NXl * j;
+  const R* W = a.win;
+
+  // halo entries of this thread: LDS slot (field * LP + ly * LX + lx, or -1)
+  // and column offset in the plane (or -1: outside the padded array). The
+  // plane size NX * NY fits a 32-bit int.
+  int h_lds[T::NHR], h_col[T::NHR];
+#pragma unroll
+  for (int r = 0; r < T::NHR; ++r) {
+    const int e = threadIdx.x + r * T::NT;
+    h_lds[r] = -1;
+    h_col[r] = -1;
+    if (e < 9 * T::NHALO) {
+      const int v = e / T::NHALO, q = e - v * T::NHALO;
+      int lx, ly;
+      T::halo_xy(q, lx, ly);
+      const int gi = H + tile_x * 64 + lx - T::XL, gj = H + tile_y * TY + ly - T::YL;
+      h_lds[r] = v * T::LP + ly * T::LX + lx;
+      if (gi >= 0 && gi < a.NX && gj >= 0 && gj < a.NY) h_col[r] = gi + a.NX * gj;
+    }
+  }
+  auto load_halo = [&](int kk, R (&hr)[T::NHR]) {
+#pragma unroll
+    for (int r = 0; r < T::NHR; ++r)
+      hr[r] = h_col[r] >= 0 ? W[(long long)(h_lds[r] / T::LP) * N + h_col[r] + (long long)kk * S] : R(0);
+  };
+
+  // z window: slots 1..4 hold planes k0 - ZL .. k0 - ZL + 3; nxt holds k0 - ZL + 4
+  R zq[9][5], nxt[9], hreg[T::NHR];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    zq[v][0] = R(0);
+#pragma unroll
+    for (int m = 1; m < 5; ++m) zq[v][m] = active ? W[(long long)v * N + col + (long long)(k0 - ZL + m - 1) * S] : R(0);
+    nxt[v] = active ? W[(long long)v * N + col + (long long)(k0 - ZL + 4) * S] : R(0);
+  }
+  load_halo(k0, hreg);
+
+  const int cidx = (ty + T::YL) * T::LX + tx + T::XL;
+  for (int k = k0; k < k1; ++k) {
+    const int buf = (k - k0) & 1;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) zq[v][m] = zq[v][m + 1];
+      zq[v][4] = nxt[v];
+    }
+    // plane k into LDS buffer `buf` (the other buffer may still be read by
+    // threads finishing plane k - 1)
+#pragma unroll
+    for (int v = 0; v < 9; ++v) sw[buf][v][cidx] = zq[v][ZL];
+#pragma unroll
+    for (int r = 0; r < T::NHR; ++r)
+      if (h_lds[r] >= 0) (&sw[buf][0][0])[h_lds[r]] = hreg[r];
+    __syncthreads();
+    // prefetch plane k + 1: window entry k + 1 - ZL + 4 and halo of k + 1
+    if (k + 1 < k1) {
+#pragma unroll
+      for (int v = 0; v < 9; ++v)
+        nxt[v] = active ? W[(long long)v * N + col + (long long)(k + 1 - ZL + 4) * S] : R(0);
+      load_halo(k + 1, hreg);
+    }
+    if (active) {
+      const R* L = &sw[buf][0][0];
+      cell_body<R, TM, FB1, FB2, FB3, STAGE>(
+          a, coef, i, j, k,
+          [&](int v) -> R {
+            const R* f = L + v * T::LP + cidx;
+            return R(FB1) * (R(AF_1) * f[-FB1] + R(AF0) * f[0] + R(AF1) * f[FB1] + R(AF2) * f[2 * FB1] +
+                             R(AF3) * f[3 * FB1]);
+          },
+          [&](int v) -> R {
+            const R* f = L + v * T::LP + cidx;
+            constexpr int s = T::LX;
+            return R(FB2) * (R(AF_1) * f[-FB2 * s] + R(AF0) * f[0] + R(AF1) * f[FB2 * s] +
+                             R(AF2) * f[2 * FB2 * s] + R(AF3) * f[3 * FB2 * s]);
+          },
+          [&](int v) -> R {
+            return R(FB3) * (R(AF_1) * zq[v][ZL - FB3] + R(AF0) * zq[v][ZL] + R(AF1) * zq[v][ZL + FB3] +
+                             R(AF2) * zq[v][ZL + 2 * FB3] + R(AF3) * zq[v][ZL + 3 * FB3]);
+          },
+          [&](int v) -> R { return zq[v][ZL]; });
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

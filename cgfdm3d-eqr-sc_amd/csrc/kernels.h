@@ -12,6 +12,9 @@ struct StageArgs {
   long long S;          // NX * NY
   int kbeg, kend;       // k range of the interior kernel [kbeg, kend)
   int stage;            // RK4 stage 0..3
+  int rty;              // interior block order: regions of 64 x (4 rty) columns per XCD
+  int kc;               // > 0: z-marching interior kernel with chunks of kc planes
+  int zty;              // z-marching tile height: 8 (512 threads) or 4 (256 threads)
   R DT, rDH;
   const R* win;         // stencil input (9 SoA fields)
   const R* wm;          // RK base (state at the start of the step)

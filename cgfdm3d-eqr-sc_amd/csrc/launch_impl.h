@@ -1,5 +1,6 @@
 // Host launchers of the CGFDM step kernels: one direct launch per F/B pattern and RK stage.
 #pragma once
+#include <type_traits>
 #include "cgfdm_step_impl.h"
 
 namespace cg {
@@ -19,6 +20,23 @@ void cell_stage(const StageArgs<R>& a, const TM* coef, dim3 g, dim3 b, hipStream
   }
 }
 
+// z-marching tiles: 64 x ty columns with ty = 8 (512 threads) or 4 (256)
+template <class R, class TM, int FB1, int FB2, int FB3, int TY>
+void zcol_stage_ty(const StageArgs<R>& a, const TM* coef, dim3 g, dim3 b, hipStream_t st) {
+  switch (a.stage) {
+    case 0: cg_zcol<R, TM, FB1, FB2, FB3, 0, TY><<<g, b, 0, st>>>(a, coef); break;
+    case 1: cg_zcol<R, TM, FB1, FB2, FB3, 1, TY><<<g, b, 0, st>>>(a, coef); break;
+    case 2: cg_zcol<R, TM, FB1, FB2, FB3, 2, TY><<<g, b, 0, st>>>(a, coef); break;
+    default: cg_zcol<R, TM, FB1, FB2, FB3, 3, TY><<<g, b, 0, st>>>(a, coef); break;
+  }
+}
+
+template <class R, class TM, int FB1, int FB2, int FB3>
+void zcol_stage(const StageArgs<R>& a, const TM* coef, dim3 g, dim3 b, hipStream_t st) {
+  if (a.zty == 4) zcol_stage_ty<R, TM, FB1, FB2, FB3, 4>(a, coef, g, b, st);
+  else zcol_stage_ty<R, TM, FB1, FB2, FB3, 8>(a, coef, g, b, st);
+}
+
 template <class R, class TM, int FB1, int FB2, int FB3>
 void surface_stage(const StageArgs<R>& a, const TM* coef, const SurfArgs<R>& s, dim3 g, dim3 b, hipStream_t st) {
   switch (a.stage) {
@@ -33,7 +51,30 @@ template <class R, class TM>
 void launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, hipStream_t st) {
   const int nk = a.kend - a.kbeg;
   if (nk <= 0 || a.nx <= 0 || a.ny <= 0 || a.stage < 0 || a.stage > 3) return;
-  const dim3 grid((a.nx + 63) / 64, (a.ny + 3) / 4, nk), block(256);
+  if constexpr (std::is_same<R, float>::value) {
+   if (a.kc > 0) {
+    // z-marching kernel (FP32 wavefield): tiles of 64 x zty columns, chunks of kc planes
+    if (a.zty != 4) a.zty = 8;
+    const long long nb = (long long)((a.nx + 63) / 64) * ((a.ny + a.zty - 1) / a.zty) * ((nk + a.kc - 1) / a.kc);
+    if (nb > 0x7fffffffLL) return;
+    const dim3 grid((unsigned)nb), block(64 * a.zty);
+    switch (fb_index(fb1, fb2, fb3)) {
+      case 0: zcol_stage<R, TM, -1, -1, -1>(a, coef, grid, block, st); break;
+      case 1: zcol_stage<R, TM, 1, -1, -1>(a, coef, grid, block, st); break;
+      case 2: zcol_stage<R, TM, -1, 1, -1>(a, coef, grid, block, st); break;
+      case 3: zcol_stage<R, TM, 1, 1, -1>(a, coef, grid, block, st); break;
+      case 4: zcol_stage<R, TM, -1, -1, 1>(a, coef, grid, block, st); break;
+      case 5: zcol_stage<R, TM, 1, -1, 1>(a, coef, grid, block, st); break;
+      case 6: zcol_stage<R, TM, -1, 1, 1>(a, coef, grid, block, st); break;
+      default: zcol_stage<R, TM, 1, 1, 1>(a, coef, grid, block, st); break;
+    }
+    return;
+   }
+  }
+  if (a.rty < 1) a.rty = 1;
+  const long long nb = interior_blocks(a.nx, a.ny, nk, a.rty);
+  if (nb > 0x7fffffffLL) return;
+  const dim3 grid((unsigned)nb), block(256);
   switch (fb_index(fb1, fb2, fb3)) {
     case 0: cell_stage<R, TM, -1, -1, -1>(a, coef, grid, block, st); break;
     case 1: cell_stage<R, TM, 1, -1, -1>(a, coef, grid, block, st); break;
