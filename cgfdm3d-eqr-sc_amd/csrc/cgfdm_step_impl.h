@@ -386,6 +386,64 @@ __global__ __launch_bounds__(64 * TY) void cg_zcol(StageArgs<R> a, const TM* __r
 }
 
 // ---------------------------------------------------------------------------
+// plane-tile interior kernel: a block of 64 x TY threads computes one 64 x TY
+// tile of one k plane. The nine fields of the tile and its x/y halo (1 cell on
+// one side, 3 on the other) are staged once into LDS, so the x and y
+// differences read LDS; the z difference reads the four other planes from
+// global memory (L2). Per cell this issues about half the vector memory
+// instructions of the thread-per-cell kernel.
+// ---------------------------------------------------------------------------
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE, int TY>
+__global__ __launch_bounds__(64 * TY) void cg_plane(StageArgs<R> a, const TM* __restrict__ coef) {
+  using T = ZTile<FB1, FB2, TY>;
+  __shared__ R sw[9][T::LP];
+  const int ntx = (a.nx + 63) / 64, nty = (a.ny + TY - 1) / TY;
+  const int b = blockIdx.x;
+  const int tile_x = b % ntx, tile_y = (b / ntx) % nty;
+  const int k = a.kbeg + b / (ntx * nty);
+  if (k >= a.kend) return;  // uniform per block, before the barrier
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int i = H + tile_x * 64 + tx, j = H + tile_y * TY + ty;
+  const bool active = i < H + a.nx && j < H + a.ny;
+  const long long N = a.N, S = a.S, NXl = a.NX;
+  const long long p = (long long)i + NXl * j + (long long)k * S;
+  const R* W = a.win;
+  const int cidx = (ty + T::YL) * T::LX + tx + T::XL;
+  R* L = &sw[0][0];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) L[v * T::LP + cidx] = active ? W[(long long)v * N + p] : R(0);
+  for (int e = threadIdx.x; e < 9 * T::NHALO; e += T::NT) {
+    const int v = e / T::NHALO, q = e - v * T::NHALO;
+    int lx, ly;
+    T::halo_xy(q, lx, ly);
+    const int gi = H + tile_x * 64 + lx - T::XL, gj = H + tile_y * TY + ly - T::YL;
+    L[v * T::LP + ly * T::LX + lx] =
+        (gi >= 0 && gi < a.NX && gj >= 0 && gj < a.NY) ? W[(long long)v * N + gi + NXl * gj + (long long)k * S] : R(0);
+  }
+  __syncthreads();
+  if (!active) return;
+  cell_body<R, TM, FB1, FB2, FB3, STAGE>(
+      a, coef, i, j, k,
+      [&](int v) -> R {
+        const R* f = L + v * T::LP + cidx;
+        return R(FB1) * (R(AF_1) * f[-FB1] + R(AF0) * f[0] + R(AF1) * f[FB1] + R(AF2) * f[2 * FB1] +
+                         R(AF3) * f[3 * FB1]);
+      },
+      [&](int v) -> R {
+        const R* f = L + v * T::LP + cidx;
+        constexpr int s = T::LX;
+        return R(FB2) * (R(AF_1) * f[-FB2 * s] + R(AF0) * f[0] + R(AF1) * f[FB2 * s] + R(AF2) * f[2 * FB2 * s] +
+                         R(AF3) * f[3 * FB2 * s]);
+      },
+      [&](int v) -> R {
+        const R* f = W + (long long)v * N + p;
+        return R(FB3) * (R(AF_1) * f[-FB3 * S] + R(AF0) * L[v * T::LP + cidx] + R(AF1) * f[FB3 * S] +
+                         R(AF2) * f[2 * FB3 * S] + R(AF3) * f[3 * FB3 * S]);
+      },
+      [&](int v) -> R { return L[v * T::LP + cidx]; });
+}
+
+// ---------------------------------------------------------------------------
 // surface kernel: top three layers, traction image
 // ---------------------------------------------------------------------------
 template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>

@@ -31,10 +31,27 @@ void zcol_stage_ty(const StageArgs<R>& a, const TM* coef, dim3 g, dim3 b, hipStr
   }
 }
 
+template <class R, class TM, int FB1, int FB2, int FB3, int TY>
+void plane_stage_ty(const StageArgs<R>& a, const TM* coef, dim3 g, dim3 b, hipStream_t st) {
+  switch (a.stage) {
+    case 0: cg_plane<R, TM, FB1, FB2, FB3, 0, TY><<<g, b, 0, st>>>(a, coef); break;
+    case 1: cg_plane<R, TM, FB1, FB2, FB3, 1, TY><<<g, b, 0, st>>>(a, coef); break;
+    case 2: cg_plane<R, TM, FB1, FB2, FB3, 2, TY><<<g, b, 0, st>>>(a, coef); break;
+    default: cg_plane<R, TM, FB1, FB2, FB3, 3, TY><<<g, b, 0, st>>>(a, coef); break;
+  }
+}
+
+// kc > 1: z-marching kernel; kc == 1: plane-tile kernel (one plane per block)
 template <class R, class TM, int FB1, int FB2, int FB3>
 void zcol_stage(const StageArgs<R>& a, const TM* coef, dim3 g, dim3 b, hipStream_t st) {
-  if (a.zty == 4) zcol_stage_ty<R, TM, FB1, FB2, FB3, 4>(a, coef, g, b, st);
-  else zcol_stage_ty<R, TM, FB1, FB2, FB3, 8>(a, coef, g, b, st);
+  if (a.kc == 1) {
+    if (a.zty == 4) plane_stage_ty<R, TM, FB1, FB2, FB3, 4>(a, coef, g, b, st);
+    else plane_stage_ty<R, TM, FB1, FB2, FB3, 8>(a, coef, g, b, st);
+  } else if (a.zty == 4) {
+    zcol_stage_ty<R, TM, FB1, FB2, FB3, 4>(a, coef, g, b, st);
+  } else {
+    zcol_stage_ty<R, TM, FB1, FB2, FB3, 8>(a, coef, g, b, st);
+  }
 }
 
 template <class R, class TM, int FB1, int FB2, int FB3>
