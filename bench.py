@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--scheme", choices=["cgfdm", "aweno"], default="cgfdm")
     ap.add_argument("--adjoint", action="store_true")
     ap.add_argument("--checkpoint", type=int, default=20)
+    ap.add_argument("--adjoint-fd", type=int, default=0, metavar="NDIR",
+                    help="with --adjoint: check the gradient against central differences in NDIR random directions")
+    ap.add_argument("--fd-steps", type=int, default=100, help="time steps of the finite-difference check")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -117,6 +120,8 @@ def main():
         return copy_calibration()
     if args.scheme == "aweno":
         return bench_aweno(args)
+    if args.adjoint and args.adjoint_fd > 0:
+        return adjoint_fd_check(args)
     if args.adjoint:
         return bench_adjoint(args)
     t0 = time.perf_counter()
@@ -263,6 +268,64 @@ def bench_adjoint(args):
         "forward_gcells": m.cells * args.steps / t_fwd / 1e9,
         "cost_ratio_fwd_adj_vs_fwd": ratio, "gradients_finite": ok,
     }
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_fd_check(args):
+    """Gradient check of the adjoint workload (256 x 256 x 128, terrain, layered +
+    random medium, PML 20, 16 x 8 finite fault, 64 stations), FP64: L2 waveform
+    misfit against a target made with a seeded 3 % Vs perturbation; gradient
+    with respect to Vs, Vp, rho per cell and the moment-rate table through the
+    torch wrapper over the HIP adjoint; central differences of the HIP forward
+    run in NDIR random directions (each perturbing all four parameter sets)."""
+    import numpy as np
+    import torch
+    from cgfdm_amd import autograd as A
+    from cgfdm_amd import model as M
+    from cgfdm_amd import solver as S
+
+    nst = args.fd_steps
+    m = M.config5(nt=nst + 1, hmin_fn=lambda c, md: S.grid_ranges(256, 256, 128, c, md)["hmin"])
+    t0 = time.perf_counter()
+    mt = M.config5(nt=nst + 1, hmin_fn=lambda c, md: S.grid_ranges(256, 256, 128, c, md)["hmin"])
+    pert = M.gaussian_random_field(mt.medium.shape[1:], mt.dh, 1000.0, 0.03, 2024)
+    mt.medium = mt.medium.copy()
+    mt.medium[0] = (mt.medium[0] * (1.0 + pert)).astype(np.float32)
+    target = S.Solver(mt, precision=8).forward_record(nst, nst)
+
+    def misfit(med, rate):
+        mm = M.config5(nt=nst + 1, hmin_fn=lambda c, md: S.grid_ranges(256, 256, 128, c, md)["hmin"])
+        mm.medium = np.ascontiguousarray(med, np.float32)
+        mm.src_rate = np.ascontiguousarray(rate, np.float32)
+        rec = S.Solver(mm, precision=8).forward_record(nst, nst)
+        return 0.5 * float(((rec - target) ** 2).sum())
+
+    ts = [torch.tensor(m.medium[c], dtype=torch.float64, requires_grad=True) for c in range(3)]
+    rate = torch.tensor(m.src_rate, dtype=torch.float64, requires_grad=True)
+    rec = A.seismograms(*ts, rate, m, nst, precision=8, checkpoint_every=max(1, nst // 10))
+    loss = 0.5 * ((rec - torch.as_tensor(target)) ** 2).sum()
+    loss.backward()
+    grads = [t.grad.numpy() for t in ts] + [rate.grad.numpy()]
+    base = [m.medium[0].astype(np.float64), m.medium[1].astype(np.float64), m.medium[2].astype(np.float64),
+            m.src_rate.astype(np.float64)]
+    rng = np.random.default_rng(11)
+    errs = []
+    for n in range(args.adjoint_fd):
+        d = [b * rng.uniform(-0.5, 0.5, b.shape) for b in base]
+        eps = 1e-3
+        vals, used = [], []
+        for sg in (1, -1):
+            pm = [(b + sg * eps * dd).astype(np.float32).astype(np.float64) for b, dd in zip(base, d)]
+            used.append(pm)
+            vals.append(misfit(np.stack(pm[:3]), pm[3]))
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        an = sum(float((g * (u0 - u1) / (2 * eps)).sum()) for g, u0, u1 in zip(grads, used[0], used[1]))
+        errs.append(abs(fd - an) / abs(fd))
+        log(f"direction {n}: finite difference {fd:.6e}, adjoint {an:.6e}, relative error {errs[-1]:.2e}")
+    out = {"check": "adjoint gradient vs central differences", "workload": f"config 5 {m.nx}x{m.ny}x{m.nz}, "
+           f"{nst} steps, FP64, misfit against a 3% Vs-perturbed target", "directions": args.adjoint_fd,
+           "max_rel_error": max(errs), "median_rel_error": float(np.median(errs)),
+           "seconds": time.perf_counter() - t0}
     print(json.dumps(out), flush=True)
 
 

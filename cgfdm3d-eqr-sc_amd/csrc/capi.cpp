@@ -29,15 +29,11 @@ int fail(int code, const std::string& msg) {
 }
 
 // Code paths that can be switched off as a whole: stepping returns
-// CGFDM_ERR_DISABLED without launching a kernel. The forward CGFDM step is on.
-// The adjoint (forward_record + adjoint) is off until its kernels, rebuilt with
-// the RK stage as a template parameter, have been executed from their compiled
-// code with every memory access checked (the run-time-stage form of the forward
-// kernels was miscompiled into a store through a stale base register).
+// CGFDM_ERR_DISABLED without launching a kernel. Both are on; the switch stays
+// so that a path can be taken out of service without touching its callers.
 constexpr bool kCgfdmStepEnabled = true;
-constexpr bool kCgfdmAdjointEnabled = false;
-const char* const kCgfdmAdjointOff =
-    "the CGFDM adjoint is switched off in this build (its kernels are not yet verified from the compiled code)";
+constexpr bool kCgfdmAdjointEnabled = true;
+const char* const kCgfdmAdjointOff = "the CGFDM adjoint is switched off in this build";
 const char* const kCgfdmStepOff = "the CGFDM step kernels are switched off in this build";
 
 #define HIPCHK(x)                                                                                  \
@@ -218,8 +214,9 @@ struct Solver : SolverBase {
   // interior block order: regions of 64 x (4 rty) columns per XCD; the
   // environment variable CGFDM_INTERIOR_RTY (1..64) overrides it for tuning
   int rty = 4;
-  // z-marching interior kernel (FP32 wavefield) with chunks of kc planes; 0
-  // selects the thread-per-cell kernel. CGFDM_INTERIOR_KC (0..4096) overrides it.
+  // interior kernel choice (FP32 wavefield): 0 thread-per-cell, -1 x-paired
+  // (FP32 coefficients, even NX), 1 plane tile, > 1 z-marching with chunks of kc
+  // planes. CGFDM_INTERIOR_KC (-1..4096) overrides it.
   int kc = 0;
   int zty = 8;  // z-marching tile height (CGFDM_INTERIOR_ZTY: 4 or 8)
 
@@ -231,7 +228,7 @@ struct Solver : SolverBase {
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_KC")) {
       const int v = std::atoi(e);
-      if (v >= 0 && v <= 4096) kc = v;
+      if (v >= -1 && v <= 4096) kc = v;
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
     NX = d.nx + 2 * cg::H;
@@ -827,8 +824,9 @@ struct AwenoSolver : SolverBase {
   // interior block order: regions of 64 x (4 rty) columns per XCD; the
   // environment variable CGFDM_INTERIOR_RTY (1..64) overrides it for tuning
   int rty = 4;
-  // z-marching interior kernel (FP32 wavefield) with chunks of kc planes; 0
-  // selects the thread-per-cell kernel. CGFDM_INTERIOR_KC (0..4096) overrides it.
+  // interior kernel choice (FP32 wavefield): 0 thread-per-cell, -1 x-paired
+  // (FP32 coefficients, even NX), 1 plane tile, > 1 z-marching with chunks of kc
+  // planes. CGFDM_INTERIOR_KC (-1..4096) overrides it.
   int kc = 0;
   int zty = 8;  // z-marching tile height (CGFDM_INTERIOR_ZTY: 4 or 8)
 
@@ -840,7 +838,7 @@ struct AwenoSolver : SolverBase {
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_KC")) {
       const int v = std::atoi(e);
-      if (v >= 0 && v <= 4096) kc = v;
+      if (v >= -1 && v <= 4096) kc = v;
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
     NX = d.nx + 2 * cg::H;

@@ -247,6 +247,134 @@ __global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restr
 }
 
 // ---------------------------------------------------------------------------
+// x-paired interior kernel (FP32 wavefield and coefficients, even NX): each
+// thread computes the two x-adjacent cells (i, i + 1), i even, so that every
+// field, coefficient and RK array access is one 8-byte load or store per
+// thread: half the vector memory instructions of the thread-per-cell kernel,
+// and the x stencil of the pair comes from four 8-byte loads per field instead
+// of ten 4-byte ones. Pairs that touch a PML slab or a ghost column take the
+// per-cell body for each of their interior cells.
+// ---------------------------------------------------------------------------
+struct __attribute__((aligned(8))) V2 {
+  float x, y;
+  __device__ V2() {}
+  __device__ V2(float s) : x(s), y(s) {}
+  __device__ V2(float a, float b) : x(a), y(b) {}
+};
+__device__ __forceinline__ V2 operator+(V2 a, V2 b) { return V2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ V2 operator-(V2 a, V2 b) { return V2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ V2 operator*(V2 a, V2 b) { return V2(a.x * b.x, a.y * b.y); }
+__device__ __forceinline__ V2& operator*=(V2& a, V2 b) { a = a * b; return a; }
+__device__ __forceinline__ V2& operator+=(V2& a, V2 b) { a = a + b; return a; }
+__device__ __forceinline__ V2 ld2(const float* p) { return *reinterpret_cast<const V2*>(p); }
+__device__ __forceinline__ void st2(float* p, V2 v) { *reinterpret_cast<V2*>(p) = v; }
+
+// 5-point difference of a pair along a non-x axis: rows f + m * FB * s, m = -1..3
+template <int FB>
+__device__ __forceinline__ V2 mac2(const float* f, long long s) {
+  return V2(float(FB)) * (V2(float(AF_1)) * ld2(f - FB * s) + V2(float(AF0)) * ld2(f) + V2(float(AF1)) * ld2(f + FB * s) +
+                          V2(float(AF2)) * ld2(f + 2 * FB * s) + V2(float(AF3)) * ld2(f + 3 * FB * s));
+}
+
+// 5-point difference of the pair (i, i + 1) along x from aligned 8-byte loads
+template <int FB>
+__device__ __forceinline__ V2 macx2(const float* f) {
+  if constexpr (FB > 0) {  // cell i: i-1 .. i+3; cell i+1: i .. i+4
+    const V2 A = ld2(f - 2), B = ld2(f), C = ld2(f + 2), D = ld2(f + 4);
+    return V2(float(AF_1) * A.y + float(AF0) * B.x + float(AF1) * B.y + float(AF2) * C.x + float(AF3) * C.y,
+              float(AF_1) * B.x + float(AF0) * B.y + float(AF1) * C.x + float(AF2) * C.y + float(AF3) * D.x);
+  } else {  // cell i: i+1 .. i-3; cell i+1: i+2 .. i-2
+    const V2 E = ld2(f - 4), A = ld2(f - 2), B = ld2(f), C = ld2(f + 2);
+    return V2(-(float(AF_1) * B.y + float(AF0) * B.x + float(AF1) * A.y + float(AF2) * A.x + float(AF3) * E.y),
+              -(float(AF_1) * C.x + float(AF0) * B.y + float(AF1) * B.x + float(AF2) * A.y + float(AF3) * A.x));
+  }
+}
+
+template <int FB1, int FB2, int FB3, int STAGE>
+__global__ __launch_bounds__(256) void cg_pair(StageArgs<float> a, const float* __restrict__ coef) {
+  const int lane = threadIdx.x & 63;
+  const int ntx = (a.nx + 1 + 127) / 128;  // 128-cell tiles covering i in [2, H + nx)
+  const int nty = (a.ny + 3) / 4;
+  const int b = blockIdx.x;
+  const int tile_x = b % ntx, tile_y = (b / ntx) % nty;
+  const int k = a.kbeg + b / (ntx * nty);
+  const int i = 2 + 2 * (64 * tile_x + lane);
+  const int j = H + 4 * tile_y + (threadIdx.x >> 6);
+  if (k >= a.kend || j >= H + a.ny) return;
+  const bool c0 = i >= H && i < H + a.nx, c1 = i + 1 < H + a.nx;
+  if (!c0 && !c1) return;
+  const long long N = a.N, S = a.S, NXl = a.NX;
+  const PmlArgs<float>& P = a.pml;
+  const float* W = a.win;
+  const bool in_pml = P.npml > 0 && (i < H + P.npml || i + 1 >= H + a.nx - P.npml || j < H + P.npml ||
+                                     j >= H + a.ny - P.npml || k < H + P.npml ||
+                                     (P.has_z2 && k >= H + a.nz - P.npml));
+  if (in_pml || !c0 || !c1) {
+#pragma unroll 1
+    for (int c = 0; c < 2; ++c) {
+      const int ic = i + c;
+      if (ic < H || ic >= H + a.nx) continue;
+      const long long p = (long long)ic + NXl * j + (long long)k * S;
+      cell_body<float, float, FB1, FB2, FB3, STAGE>(
+          a, coef, ic, j, k, [&](int v) -> float { return mac<FB1>(W + (long long)v * N + p, 1); },
+          [&](int v) -> float { return mac<FB2>(W + (long long)v * N + p, NXl); },
+          [&](int v) -> float { return mac<FB3>(W + (long long)v * N + p, S); },
+          [&](int v) -> float { return W[(long long)v * N + p]; });
+    }
+    return;
+  }
+  const long long p = (long long)i + NXl * j + (long long)k * S;  // even: 8-byte aligned
+  auto cf = [&](int c) -> V2 { return ld2(coef + (long long)c * N + p); };
+  const V2 mu = cf(M_MU), lam = cf(M_LAM), bu = cf(M_B);
+  const V2 rdh(a.rDH), dt(a.DT);
+  V2 h[9], D[9], Q[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) D[v] = macx2<FB1>(W + (long long)v * N + p);
+  assemble(D, cf(M_XIX), cf(M_XIY), cf(M_XIZ), mu, lam, bu, Q);
+  const V2 bx(P.beta[0][i], P.beta[0][i + 1]);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    Q[v] *= rdh;
+    h[v] = bx * Q[v] * dt;
+  }
+#pragma unroll
+  for (int v = 0; v < 9; ++v) D[v] = mac2<FB2>(W + (long long)v * N + p, NXl);
+  assemble(D, cf(M_ETX), cf(M_ETY), cf(M_ETZ), mu, lam, bu, Q);
+  const V2 by(P.beta[1][j]);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    Q[v] *= rdh;
+    h[v] += by * Q[v] * dt;
+  }
+#pragma unroll
+  for (int v = 0; v < 9; ++v) D[v] = mac2<FB3>(W + (long long)v * N + p, S);
+  assemble(D, cf(M_ZTX), cf(M_ZTY), cf(M_ZTZ), mu, lam, bu, Q);
+  const V2 bz(P.beta[2][k]);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    Q[v] *= rdh;
+    h[v] += bz * Q[v] * dt;
+  }
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    const long long q = (long long)v * N + p;
+    if constexpr (STAGE == 0) {
+      const V2 m = ld2(W + q);
+      st2(a.wt + q, m + V2(float(RK_B1)) * h[v]);
+      st2(a.wout + q, m + V2(float(RK_A2)) * h[v]);
+    } else if constexpr (STAGE == 1) {
+      st2(a.wt + q, ld2(a.wt + q) + V2(float(RK_B2)) * h[v]);
+      st2(a.wout + q, ld2(a.wm + q) + V2(float(RK_A3)) * h[v]);
+    } else if constexpr (STAGE == 2) {
+      st2(a.wt + q, ld2(a.wt + q) + V2(float(RK_B3)) * h[v]);
+      st2(a.wout + q, ld2(a.wm + q) + h[v]);
+    } else {
+      st2(a.wout + q, ld2(a.wt + q) + V2(float(RK_B4)) * h[v]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // z-marching interior kernel: a block of 64 x TY threads owns 64 x TY columns
 // and walks a chunk of kc planes up z. Per plane it reads the nine fields once:
 // the block's own cells into a per-thread register window over z (planes

@@ -41,6 +41,16 @@ void plane_stage_ty(const StageArgs<R>& a, const TM* coef, dim3 g, dim3 b, hipSt
   }
 }
 
+template <int FB1, int FB2, int FB3>
+void pair_stage(const StageArgs<float>& a, const float* coef, dim3 g, dim3 b, hipStream_t st) {
+  switch (a.stage) {
+    case 0: cg_pair<FB1, FB2, FB3, 0><<<g, b, 0, st>>>(a, coef); break;
+    case 1: cg_pair<FB1, FB2, FB3, 1><<<g, b, 0, st>>>(a, coef); break;
+    case 2: cg_pair<FB1, FB2, FB3, 2><<<g, b, 0, st>>>(a, coef); break;
+    default: cg_pair<FB1, FB2, FB3, 3><<<g, b, 0, st>>>(a, coef); break;
+  }
+}
+
 // kc > 1: z-marching kernel; kc == 1: plane-tile kernel (one plane per block)
 template <class R, class TM, int FB1, int FB2, int FB3>
 void zcol_stage(const StageArgs<R>& a, const TM* coef, dim3 g, dim3 b, hipStream_t st) {
@@ -68,6 +78,25 @@ template <class R, class TM>
 void launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, hipStream_t st) {
   const int nk = a.kend - a.kbeg;
   if (nk <= 0 || a.nx <= 0 || a.ny <= 0 || a.stage < 0 || a.stage > 3) return;
+  if constexpr (std::is_same<R, float>::value && std::is_same<TM, float>::value) {
+    if (a.kc < 0 && a.NX % 2 == 0) {
+      // x-paired kernel: 128 x 4 cells per block, 8-byte accesses
+      const long long nb = (long long)((a.nx + 1 + 127) / 128) * ((a.ny + 3) / 4) * nk;
+      if (nb > 0x7fffffffLL) return;
+      const dim3 grid((unsigned)nb), block(256);
+      switch (fb_index(fb1, fb2, fb3)) {
+        case 0: pair_stage<-1, -1, -1>(a, coef, grid, block, st); break;
+        case 1: pair_stage<1, -1, -1>(a, coef, grid, block, st); break;
+        case 2: pair_stage<-1, 1, -1>(a, coef, grid, block, st); break;
+        case 3: pair_stage<1, 1, -1>(a, coef, grid, block, st); break;
+        case 4: pair_stage<-1, -1, 1>(a, coef, grid, block, st); break;
+        case 5: pair_stage<1, -1, 1>(a, coef, grid, block, st); break;
+        case 6: pair_stage<-1, 1, 1>(a, coef, grid, block, st); break;
+        default: pair_stage<1, 1, 1>(a, coef, grid, block, st); break;
+      }
+      return;
+    }
+  }
   if constexpr (std::is_same<R, float>::value) {
    if (a.kc > 0) {
     // z-marching kernel (FP32 wavefield): tiles of 64 x zty columns, chunks of kc planes

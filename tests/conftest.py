@@ -3,13 +3,6 @@ import sys
 
 import pytest
 
-# The CGFDM adjoint (forward_record + adjoint) is switched off in the library
-# (CGFDM_ERR_DISABLED) until its kernels, rebuilt with the RK stage as a template
-# parameter, have been executed from their compiled code with every access
-# checked. The GPU tests that call it carry this mark.
-CGFDM_ADJOINT_OFF = pytest.mark.skip(
-    reason="CGFDM adjoint switched off until its rebuilt kernels are verified from the compiled code")
-
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "cgfdm3d-eqr-sc_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -11,9 +11,7 @@ import pytest
 
 from cgfdm_amd import model as M
 
-from conftest import CGFDM_ADJOINT_OFF
-
-pytestmark = [pytest.mark.gpu, CGFDM_ADJOINT_OFF]
+pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")

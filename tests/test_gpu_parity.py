@@ -116,17 +116,6 @@ def test_16bit_coefficients_vs_fp64_oracle(coef, tol, oracle_mod, solver_mod):
     assert rel_l2(g.pgv(), o64.pgv()) < tol
 
 
-def test_adjoint_is_switched_off(solver_mod):
-    """forward_record fails loudly (no kernel is launched) while the adjoint is
-    switched off, and leaves the solver where it was."""
-    m = make("no_pml_flat")
-    g = solver_mod.Solver(m, precision=8)
-    with pytest.raises(solver_mod.CgfdmError) as e:
-        g.forward_record(2, 1)
-    assert e.value.code == 4
-    assert g.it == 0
-
-
 def test_error_codes(solver_mod):
     m = make("no_pml_flat")
     m.dt = m.dt * 10.0
