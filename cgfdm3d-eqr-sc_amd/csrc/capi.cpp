@@ -210,6 +210,10 @@ struct Solver : SolverBase {
   R* adj_seg = nullptr;
   R* adj_src = nullptr;
   R* adj_end = nullptr;
+  // stage states X0..X3 of every step of one checkpoint segment (4 x 9N per
+  // step), kept when they fit CGFDM_ADJOINT_STORE_GB (default 64 GiB): the
+  // adjoint then reads them instead of recomputing three stages per step
+  R* adj_stages = nullptr;
 
   // interior block order: regions of 64 x (4 rty) columns per XCD; the
   // environment variable CGFDM_INTERIOR_RTY (1..64) overrides it for tuning
@@ -612,11 +616,49 @@ struct Solver : SolverBase {
   }
   void release_adjoint() {
     for (void* q : {(void*)ck_store, (void*)rec_vel, (void*)adj_buf, (void*)adj_face, (void*)adj_grad,
-                    (void*)adj_gsrc, (void*)adj_seg, (void*)adj_src, (void*)adj_end})
+                    (void*)adj_gsrc, (void*)adj_seg, (void*)adj_src, (void*)adj_end, (void*)adj_stages})
       if (q) dfree(q);
-    ck_store = rec_vel = adj_buf = adj_face = adj_seg = adj_src = adj_end = nullptr;
+    ck_store = rec_vel = adj_buf = adj_face = adj_seg = adj_src = adj_end = adj_stages = nullptr;
     adj_grad = adj_gsrc = nullptr;
     fr_n = 0;
+  }
+  // source table index and weight at step index `step` (addMoment.cpp:262-377), -1 if none
+  int source_index(int step, float& tw) const {
+    if (src_np <= 0) return -1;
+    const float DTf = (float)d.dt;
+    const float tmpT = ((float)step + 0.0f) * DTf;
+    const int srcIt = int(tmpT / (float)src_dt);
+    if (srcIt + 1 >= src_nt) return -1;
+    const float t1 = float(srcIt) * (float)src_dt;
+    tw = (tmpT - t1) / (float)src_dt;
+    return srcIt;
+  }
+  // one forward step (no observers) that leaves its stage states in x[0..3]:
+  // x0 = W after the source, x1..x3 = inputs of stages 1..3
+  int step_store(R* x0, R* x1, R* x2, R* x3) {
+    const int kend = cg::H + d.nz - (d.free_surface ? 3 : 0);
+    int fb[3] = {FBTAB[it % 8][0], FBTAB[it % 8][1], FBTAB[it % 8][2]};
+    float tw = 0.f;
+    const int srcIt = source_index(it, tw);
+    if (srcIt >= 0) cg::launch_source<R>(A, N, src_np, src_cell, src_rate, srcIt, R(tw), R(d.dt), stream);
+    HIPCHK(hipMemcpyAsync(x0, A, sizeof(R) * 9 * N, hipMemcpyDeviceToDevice, stream));
+    R* ins[4] = {A, x1, x2, x3};
+    R* outs[4] = {x1, x2, x3, A};
+    for (int irk = 0; irk < 4; ++irk) {
+      cg::StageArgs<R> a{};
+      a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.NX = NX; a.NY = NY; a.N = N; a.S = S;
+      a.kbeg = cg::H; a.kend = kend; a.stage = irk; a.rty = rty; a.kc = kc; a.zty = zty;
+      a.DT = R(d.dt); a.rDH = R(1.0 / d.dh);
+      a.win = ins[irk]; a.wout = outs[irk]; a.wm = A; a.wt = T; a.pml = pml;
+      cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream);
+      if (d.free_surface) {
+        cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, nullptr, dis};
+        cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], stream);
+      }
+      fb[0] = -fb[0]; fb[1] = -fb[1]; fb[2] = -fb[2];
+    }
+    ++it;
+    return CGFDM_OK;
   }
   // source on A at step index `step` (addMoment.cpp:262-377); returns srcIt (-1 if none)
   int inject_source(int step, float& tw) {
@@ -723,6 +765,12 @@ struct Solver : SolverBase {
         off += 3 * c;
       }
     }
+    if (!adj_stages) {
+      double gib = 64.0;
+      if (const char* e = std::getenv("CGFDM_ADJOINT_STORE_GB")) gib = std::atof(e);
+      const double need = (double)sizeof(R) * 36.0 * (double)N * fr_ck;
+      if (need <= gib * 1073741824.0 && dalloc(&adj_stages, 36LL * N * fr_ck) != CGFDM_OK) adj_stages = nullptr;
+    }
     if ((rc = copy_state(adj_end, true))) return rc;
     const int nck = (fr_n + fr_ck - 1) / fr_ck;
     for (int seg = nck - 1; seg >= 0; --seg) {
@@ -730,16 +778,30 @@ struct Solver : SolverBase {
       if ((rc = copy_state(ck_store + (long long)seg * ss, false))) return rc;
       it = fr_it0 + n0;
       for (int n = n0; n < n1; ++n) {
-        if ((rc = copy_state(adj_seg + (long long)(n - n0) * ss, true))) return rc;
-        if ((rc = step_impl(1, false))) return rc;
+        if (adj_stages) {
+          R* xs = adj_stages + 36LL * N * (n - n0);
+          if ((rc = step_store(xs, xs + 9 * N, xs + 18 * N, xs + 27 * N))) return rc;
+        } else {
+          if ((rc = copy_state(adj_seg + (long long)(n - n0) * ss, true))) return rc;
+          if ((rc = step_impl(1, false))) return rc;
+        }
       }
       for (int n = n1 - 1; n >= n0; --n) {
         cg::launch_adj_inject<R>(MU, N, ns, st_cell, adj_src, fr_n, n, stream);
-        if ((rc = copy_state(adj_seg + (long long)(n - n0) * ss, false))) return rc;
         const int step = fr_it0 + n;
         float tw = 0.f;
-        const int srcIt = inject_source(step, tw);
-        stage_states(step, X3);
+        int srcIt;
+        R* X[4];
+        if (adj_stages) {
+          R* xs = adj_stages + 36LL * N * (n - n0);
+          for (int q = 0; q < 4; ++q) X[q] = xs + 9LL * N * q;
+          srcIt = source_index(step, tw);
+        } else {
+          if ((rc = copy_state(adj_seg + (long long)(n - n0) * ss, false))) return rc;
+          srcIt = inject_source(step, tw);
+          stage_states(step, X3);
+          X[0] = A; X[1] = B; X[2] = C; X[3] = X3;
+        }
         const int* fb0 = FBTAB[step % 8];
         for (int s = 3; s >= 0; --s) {
           const int sg = (s % 2 == 0) ? 1 : -1;
@@ -749,7 +811,7 @@ struct Solver : SolverBase {
           a.stage = s;
           a.DT = R(d.dt);
           a.rDH = R(1.0 / d.dh);
-          a.X = s == 0 ? A : s == 1 ? B : s == 2 ? C : X3;
+          a.X = X[s];
           a.mu = MU; a.out = OUT; a.G = G; a.Z = Z; a.Ls = Ls;
           a.gmu = adj_grad; a.glam = adj_grad + N; a.gb = adj_grad + 2 * N;
           a.pml = pml;

@@ -126,3 +126,13 @@ def test_hip_gradients_match_finite_differences_of_the_hip_forward(solver_mod):
         fd = (vals[0] - vals[1]) / (2 * eps)
         an = float((grads[k] * (used[0] - used[1]) / (2 * eps)).sum())
         assert abs(fd - an) <= 1e-5 * abs(fd), (k, fd, an)
+
+
+def test_adjoint_with_recomputed_stages_matches_oracle(oracle_mod, solver_mod, monkeypatch):
+    """With no memory budget for the stage states of a segment the adjoint
+    recomputes them per step; the gradients are the same."""
+    monkeypatch.setenv("CGFDM_ADJOINT_STORE_GB", "0")
+    m, g, rec, gr, r = run("pml_fs_curvilinear", 8, oracle_mod, solver_mod)
+    for k in ("gmu", "glam", "gb", "grate"):
+        scale = np.abs(r[k]).max()
+        np.testing.assert_allclose(gr[k], r[k], rtol=0, atol=1e-10 * scale, err_msg=k)
