@@ -34,6 +34,27 @@ def _compile(src):
     return out, errs
 
 
+HOST = os.path.join(ROOT, "host")
+HOST_SOURCES = ["json_lite.cpp", "params.cpp", "model_build.cpp", "source_build.cpp", "io_files.cpp"]
+RUNNER = os.path.join(PKG, "cgfdm_run")
+
+
+def build_runner(verbose=True):
+    """The params.json driver (C++ host on the C ABI), linked against the library
+    next to it; the CPU-only host sources are also used by the host tests."""
+    srcs = [os.path.join(HOST, f) for f in HOST_SOURCES + ["cgfdm_run.cpp"]]
+    tmp = RUNNER + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, "-I", HOST, "-o", tmp] + srcs + \
+          ["-L", PKG, "-lcgfdm_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed for cgfdm_run:\n" + r.stderr)
+    os.replace(tmp, RUNNER)
+    if verbose:
+        print("built", RUNNER, file=sys.stderr)
+    return RUNNER
+
+
 def build(verbose=True, jobs=None):
     os.makedirs(OBJ, exist_ok=True)
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 4)
@@ -47,6 +68,7 @@ def build(verbose=True, jobs=None):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB, file=sys.stderr)
+    build_runner(verbose)
     return LIB
 
 

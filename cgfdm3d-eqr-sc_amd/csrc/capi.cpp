@@ -67,12 +67,56 @@ struct SolverBase {
   virtual int get_field(int comp, double* out) = 0;
   virtual int set_field(int comp, const double* in) = 0;
   virtual int get_setup(int what, double* out) = 0;
+  // one component of the current state on an interior plane, physical units
+  virtual int get_slice(int comp, int axis, int index, float* out) = 0;
   virtual int forward_record(int, int, double*) { return fail(CGFDM_ERR_ARG, "the adjoint is available for the CGFDM scheme"); }
   virtual int adjoint(const double*, double*, double*, double*, double*) {
     return fail(CGFDM_ERR_ARG, "the adjoint is available for the CGFDM scheme");
   }
   hipStream_t stream = nullptr;
   int it = 0;
+  // padded extents as seen by the plane copies (set by the solvers at init)
+  int pNX = 0, pNY = 0, pNZ = 0;
+
+  // Strided device-to-host copy of the interior of one padded plane of a
+  // plane-major array (x fastest): axis 0 = the y-z plane at i = index
+  // ([nz][ny]), 1 = the x-z plane at j = index ([nz][nx]), 2 = the x-y plane at
+  // k = index ([ny][nx]). No kernel: hipMemcpy2D with the plane's stride.
+  template <class T>
+  int copy_plane(const T* plane0, int axis, int index, std::vector<T>& out) {
+    const int H = 3;
+    const size_t e = sizeof(T);
+    const long long S2 = (long long)pNX * pNY;
+    int na, nb;
+    if (axis == 0) {
+      na = pNY - 2 * H;
+      nb = pNZ - 2 * H;
+      out.resize((size_t)na * nb);
+      // one element per padded row, rows j = H.. of planes k = H..
+      for (int b = 0; b < nb; ++b)
+        HIPCHK(hipMemcpy2D(out.data() + (size_t)b * na, e, plane0 + index + (long long)pNX * H + S2 * (b + H),
+                           e * pNX, e, na, hipMemcpyDeviceToHost));
+    } else if (axis == 1) {
+      na = pNX - 2 * H;
+      nb = pNZ - 2 * H;
+      out.resize((size_t)na * nb);
+      HIPCHK(hipMemcpy2D(out.data(), e * na, plane0 + H + (long long)pNX * index + S2 * H, e * S2, e * na, nb,
+                         hipMemcpyDeviceToHost));
+    } else {
+      na = pNX - 2 * H;
+      nb = pNY - 2 * H;
+      out.resize((size_t)na * nb);
+      HIPCHK(hipMemcpy2D(out.data(), e * na, plane0 + H + (long long)pNX * H + S2 * index, e * pNX, e * na, nb,
+                         hipMemcpyDeviceToHost));
+    }
+    return CGFDM_OK;
+  }
+  int check_plane(int comp, int axis, int index) {
+    const int n = axis == 0 ? pNX : axis == 1 ? pNY : pNZ;
+    if (comp < 0 || comp > 8 || axis < 0 || axis > 2 || index < 3 || index >= n - 3)
+      return fail(CGFDM_ERR_ARG, "slice component, axis or index out of range");
+    return CGFDM_OK;
+  }
   double ranges[8] = {0};
   long long bytes = 0;
 
@@ -240,6 +284,9 @@ struct Solver : SolverBase {
     NZ = d.nz + 2 * cg::H;
     S = (long long)NX * NY;
     N = S * NZ;
+    pNX = NX;
+    pNY = NY;
+    pNZ = NZ;
     int rc;
     // --- temporaries: coordinates, medium, FP64 metrics
     float *dcoord = nullptr, *dmed = nullptr;
@@ -543,6 +590,16 @@ struct Solver : SolverBase {
     HIPCHK(hipMemcpy(h.data(), A + (size_t)comp * N, sizeof(R) * N, hipMemcpyDeviceToHost));
     double sc = comp >= 3 ? Z : 1.0;
     for (long long p = 0; p < N; ++p) out[p] = (double)h[p] * sc;
+    return CGFDM_OK;
+  }
+
+  int get_slice(int comp, int axis, int index, float* out) override {
+    if (int rc = check_plane(comp, axis, index)) return rc;
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<R> h;
+    if (int rc = copy_plane(A + (size_t)comp * N, axis, index, h)) return rc;
+    const double sc = comp >= 3 ? Z : 1.0;
+    for (size_t p = 0; p < h.size(); ++p) out[p] = (float)((double)h[p] * sc);
     return CGFDM_OK;
   }
 
@@ -908,6 +965,9 @@ struct AwenoSolver : SolverBase {
     NZ = d.nz + 2 * cg::H;
     S = (long long)NX * NY;
     N = S * NZ;
+    pNX = NX;
+    pNY = NY;
+    pNZ = NZ;
     int rc;
     float *dcoord = nullptr, *dmed = nullptr;
     double* cov = nullptr;
@@ -1102,6 +1162,42 @@ struct AwenoSolver : SolverBase {
     return CGFDM_OK;
   }
 
+  // physical (Vx, Vy, Vz, Txx, Tyy, Tzz, Txy, Txz, Tyz) from the conservative
+  // state (rho v, strains) and mu, lambda, 1/rho (wave_conserv2phy,
+  // data_io.cpp:435-483)
+  int get_slice(int comp, int axis, int index, float* out) override {
+    if (int rc = check_plane(comp, axis, index)) return rc;
+    HIPCHK(hipStreamSynchronize(stream));
+    const R* W = buf[cur];
+    std::vector<R> u0, u1, u2, mu, lam, b;
+    int rc;
+    if (comp < 3) {
+      if ((rc = copy_plane(W + (size_t)comp * N, axis, index, u0))) return rc;
+      if ((rc = copy_plane(cjm + (size_t)aw::C_B * N, axis, index, b))) return rc;
+      for (size_t p = 0; p < u0.size(); ++p) out[p] = (float)(u0[p] * b[p]);
+      return CGFDM_OK;
+    }
+    if ((rc = copy_plane(cjm + (size_t)aw::C_MU * N, axis, index, mu))) return rc;
+    if (comp >= 6) {
+      // Txy = mu gxy (8), Txz = mu gxz (7), Tyz = mu gyz (6)
+      if ((rc = copy_plane(W + (size_t)(14 - comp) * N, axis, index, u0))) return rc;
+      for (size_t p = 0; p < u0.size(); ++p) out[p] = (float)(mu[p] * u0[p]);
+      return CGFDM_OK;
+    }
+    if ((rc = copy_plane(cjm + (size_t)aw::C_LAM * N, axis, index, lam))) return rc;
+    if ((rc = copy_plane(W + (size_t)3 * N, axis, index, u0))) return rc;
+    if ((rc = copy_plane(W + (size_t)4 * N, axis, index, u1))) return rc;
+    if ((rc = copy_plane(W + (size_t)5 * N, axis, index, u2))) return rc;
+    for (size_t p = 0; p < u0.size(); ++p) {
+      const R l = lam[p], m2 = l + 2 * mu[p];
+      const R v = comp == 3 ? l * u1[p] + l * u2[p] + u0[p] * m2
+                : comp == 4 ? l * u0[p] + l * u2[p] + u1[p] * m2
+                            : l * u0[p] + l * u1[p] + u2[p] * m2;
+      out[p] = (float)v;
+    }
+    return CGFDM_OK;
+  }
+
   int set_field(int comp, const double* in) override {
     if (comp < 0 || comp > 8) return fail(CGFDM_ERR_ARG, "component out of range");
     HIPCHK(hipStreamSynchronize(stream));
@@ -1268,6 +1364,11 @@ int cgfdm_get_pgv(cgfdm_solver* s, float* out) {
 int cgfdm_get_surface_displacement(cgfdm_solver* s, float* out) {
   SOLVER_OR_FAIL(s);
   return s->impl->get_dis(out);
+}
+int cgfdm_get_slice(cgfdm_solver* s, int comp, int axis, int index, float* out) {
+  SOLVER_OR_FAIL(s);
+  if (!out) return fail(CGFDM_ERR_ARG, "null output");
+  return s->impl->get_slice(comp, axis, index, out);
 }
 int cgfdm_get_field(cgfdm_solver* s, int comp, double* out) {
   SOLVER_OR_FAIL(s);

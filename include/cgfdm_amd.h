@@ -101,6 +101,14 @@ int cgfdm_get_surface_displacement(cgfdm_solver* s, float* out);
  * Tyy,Tzz,Txy,Txz,Tyz); AWENO: the conservative state (rho vx, rho vy, rho vz,
  * exx, eyy, ezz, gyz, gxz, gxy) (station records and PGV are physical for both). */
 int cgfdm_get_field(cgfdm_solver* s, int comp, double* out);
+/* One wavefield component of the current state on an interior plane, float in
+ * physical units (Vx,Vy,Vz,Txx,Tyy,Tzz,Txy,Txz,Tyz for both schemes; AWENO converts
+ * its conservative state as wave_conserv2phy, data_io.cpp:435-483). axis 0: the y-z
+ * plane at padded i = index, float[nz][ny]; axis 1: the x-z plane at padded j,
+ * float[nz][nx]; axis 2: the x-y plane at padded k, float[ny][nx]. The slice and
+ * free-surface outputs of data2D_XYZ_out (data_io.cpp:485-587, pack_iodata_x/y/z
+ * :96-245) read these planes. */
+int cgfdm_get_slice(cgfdm_solver* s, int comp, int axis, int index, float* out);
 /* Overwrite one wavefield component (initial conditions; no reference counterpart). */
 int cgfdm_set_field(cgfdm_solver* s, int comp, const double* in);
 

@@ -1,0 +1,148 @@
+"""The params.json driver (cgfdm_run) on the MI355X against the CPU oracle.
+
+cgfdm_run reads params.json/station.json, builds the grid, medium and source,
+steps the HIP library and writes the reference's files split over a 2 x 1 x 2
+rank layout. The same inputs (grid and medium from the driver's own builders,
+the source table from its single-source builder with J and mu of the oracle's
+setup) are stepped by the FP64 oracle; the files, reassembled as the
+reference's plotting scripts do, must match it: station seismograms, the PGV
+map and the velocity slices within 1e-4 relative L2 (FP32 GPU vs FP64 oracle),
+for the CGFDM and the ME-AWENO builds.
+"""
+import ctypes
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cgfdm_amd import model as M
+import test_host_driver as T
+
+pytestmark = pytest.mark.gpu
+H = M.HALO
+NX_, NY_, NZ_ = 28, 24, 20
+PXYZ = (2, 1, 2)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def setup_case(tmp_path, scheme):
+    lay = tmp_path / "layers.bin"
+    rows = [(0.0, 2.0, 3.6, 2.2), (0.4, 3.0, 5.2, 2.6), (1.0, 3.4, 6.0, 2.8)]
+    with open(lay, "wb") as f:
+        f.write(np.int32(len(rows)).tobytes())
+        f.write(np.array(rows, np.float32).tobytes())
+    p = dict(TMAX=0.3, DT=0.008, DH=100.0, NX=NX_, NY=NY_, NZ=NZ_, PX=PXYZ[0], PY=PXYZ[1], PZ=PXYZ[2], centerX=14,
+             centerY=12, IT_SKIP=10, sliceX=9, sliceY=12, sliceZ=14, sliceFreeSurf=1, nPML=5, gauss_hill=1,
+             useTerrain=0, useMedium=1, LayeredModel=1, LayeredFileName=str(lay), useMultiSource=0, sourceX=14,
+             sourceY=12, sourceZ=12, rickerfc=4.0, out="./out")
+    p["useSingleSource(ricker)"] = 1
+    (tmp_path / "params.json").write_text(json.dumps(p))
+    st = {"0": [7, 6, NZ_ - 1], "1": [20, 6, NZ_ - 1], "2": [7, 18, NZ_ - 1], "3": [21, 17, NZ_ - 1],
+          "4": [14, 12, 8], "5": [3, 20, NZ_ - 1]}
+    (tmp_path / "station.json").write_text(json.dumps({"point": 1, "station(point)": st}))
+    return p, np.array(list(st.values()), np.int32)
+
+
+def run_driver(tmp_path, scheme):
+    cmd = [T.RUNNER, "--quiet", "--scheme", scheme]
+    print("driver:", " ".join(cmd), "in", tmp_path, flush=True)
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def oracle_model(lib, tmp_path, scheme, oracle_mod, nt):
+    nx, ny, nz = NX_, NY_, NZ_
+    rc, coord, terrain, medium = T.model(lib, str(tmp_path / "params.json"), nx, ny, nz)
+    assert rc == 0
+    m = M.Model(nx=nx, ny=ny, nz=nz, dh=100.0, dt=float(np.float32(0.008)), nt=nt, npml=5, coord=coord,
+                medium=medium, free_surface=True)
+    cls = oracle_mod.Oracle if scheme == "cgfdm" else oracle_mod.AwenoOracle
+    probe = cls(m, prec=8, threads=8)
+    jac, mu = probe.setup(9), probe.setup(10)
+    del probe
+    rc, cells, rate, dt = T.single(lib, str(tmp_path / "params.json"), nt, np.ascontiguousarray(jac),
+                                   np.ascontiguousarray(mu))
+    assert rc == 0
+    m.src_cells, m.src_rate, m.src_dt = cells, rate, dt
+    return m, cls
+
+
+def gather(path_fmt, spans_a, spans_b, ranks, shape, fast_axis, slow_axis):
+    """Reassemble rank files into the whole plane (plotXY.py:80-112)."""
+    out = np.zeros(shape, np.float32)
+    found = 0
+    for (X, Y, Z) in ranks:
+        f = path_fmt.format(X=X, Y=Y, Z=Z)
+        if not os.path.exists(f):
+            continue
+        found += 1
+        idx = {"X": X, "Y": Y, "Z": Z}
+        a0, an = spans_a[idx[fast_axis]]
+        b0, bn = spans_b[idx[slow_axis]]
+        out[b0:b0 + bn, a0:a0 + an] = np.fromfile(f, np.float32).reshape(bn, an)
+    return out, found
+
+
+@pytest.mark.parametrize("scheme", ["cgfdm", "aweno"])
+def test_driver_outputs_match_oracle(scheme, tmp_path, oracle_mod):
+    lib = T.host_lib()
+    p, st = setup_case(tmp_path, scheme)
+    nt = int(0.3 / float(np.float32(0.008)))
+    run_driver(tmp_path, scheme)
+    out = tmp_path / "out"
+    m, cls = oracle_model(lib, tmp_path, scheme, oracle_mod, nt)
+    m.stations = (st + H).astype(np.int32)
+    o = cls(m, prec=8, threads=8)
+    sx, sy, sz = T.spans(NX_, PXYZ[0]), T.spans(NY_, PXYZ[1]), T.spans(NZ_, PXYZ[2])
+    ranks = [(X, Y, Z) for X in range(PXYZ[0]) for Y in range(PXYZ[1]) for Z in range(PXYZ[2])]
+    # velocity slices after step 20 (file index 20) and the free-surface slice
+    o.step(21)
+    vx = o.field(0)
+    xs, n1 = gather(str(out / "Vx_20_X_mpi_{X}_{Y}_{Z}.bin"), sy, sz, ranks, (NZ_, NY_), "Y", "Z")
+    zs, n2 = gather(str(out / "Vx_20_Z_mpi_{X}_{Y}_{Z}.bin"), sx, sy, ranks, (NY_, NX_), "X", "Y")
+    fs, n3 = gather(str(out / "FreeSurfVx_20_Z_mpi_{X}_{Y}_{Z}.bin"), sx, sy, ranks, (NY_, NX_), "X", "Y")
+    assert n1 == 2 and n2 == 2 and n3 == 2
+    if scheme == "aweno":
+        # the AWENO oracle holds rho v: velocity = (rho v) / rho
+        vx = vx * o.setup(12)
+    assert rel_l2(xs, vx[H:-H, H:-H, H + 9]) < 1e-4
+    assert rel_l2(zs, vx[H + 14, H:-H, H:-H]) < 1e-4
+    assert rel_l2(fs, vx[H + NZ_ - 1, H:-H, H:-H]) < 1e-4
+    o.step(nt - 21)
+    # station files: indices and seismograms of the stations each rank holds
+    so = o.stations()
+    got = {}
+    for (X, Y, Z) in ranks:
+        f = out / f"station_mpi_{X}_{Y}_{Z}.bin"
+        if not f.exists():
+            continue
+        raw = f.read_bytes()
+        n = (len(raw)) // (12 + 4 * nt * 9)
+        xyz = np.frombuffer(raw[:12 * n], np.int32).reshape(n, 3)
+        wave = np.frombuffer(raw[12 * n:], np.float32).reshape(n, nt, 9)
+        for q in range(n):
+            got[tuple(xyz[q])] = wave[q]
+    assert len(got) == len(st)
+    for s, (i, j, k) in enumerate(st):
+        w = got[(i, j, k)]
+        for fam in (slice(0, 3), slice(3, 9)):
+            assert rel_l2(w[:, fam], so[s, :, fam]) < 1e-4, (scheme, s, fam)
+    # PGV map of the surface ranks
+    pg = np.zeros((NY_, NX_, 2), np.float32)
+    for (X, Y, Z) in ranks:
+        f = out / f"PGV_Z_mpi_{X}_{Y}_{Z}.bin"
+        assert f.exists() == (Z == PXYZ[2] - 1)
+        if f.exists():
+            pg[sy[Y][0]:sum(sy[Y]), sx[X][0]:sum(sx[X])] = np.fromfile(f, np.float32).reshape(sy[Y][1], sx[X][1], 2)
+    assert rel_l2(pg, o.pgv()) < 1e-4
+    # model slices written at setup
+    vs, nv = gather(str(out / "Vs_Z_mpi_{X}_{Y}_{Z}.bin"), sx, sy, ranks, (NY_, NX_), "X", "Y")
+    assert nv == 2 and np.array_equal(vs, m.medium[0, H + 14, H:-H, H:-H])
