@@ -75,8 +75,9 @@ struct SolverBase {
   }
   hipStream_t stream = nullptr;
   int it = 0;
-  // padded extents as seen by the plane copies (set by the solvers at init)
-  int pNX = 0, pNY = 0, pNZ = 0;
+  // padded extents as seen by the plane copies (set by the solvers at init):
+  // pNX is the device row pitch, pXe the padded x extent (nx + 6)
+  int pNX = 0, pNY = 0, pNZ = 0, pXe = 0;
 
   // Strided device-to-host copy of the interior of one padded plane of a
   // plane-major array (x fastest): axis 0 = the y-z plane at i = index
@@ -97,13 +98,13 @@ struct SolverBase {
         HIPCHK(hipMemcpy2D(out.data() + (size_t)b * na, e, plane0 + index + (long long)pNX * H + S2 * (b + H),
                            e * pNX, e, na, hipMemcpyDeviceToHost));
     } else if (axis == 1) {
-      na = pNX - 2 * H;
+      na = pXe - 2 * H;
       nb = pNZ - 2 * H;
       out.resize((size_t)na * nb);
       HIPCHK(hipMemcpy2D(out.data(), e * na, plane0 + H + (long long)pNX * index + S2 * H, e * S2, e * na, nb,
                          hipMemcpyDeviceToHost));
     } else {
-      na = pNX - 2 * H;
+      na = pXe - 2 * H;
       nb = pNY - 2 * H;
       out.resize((size_t)na * nb);
       HIPCHK(hipMemcpy2D(out.data(), e * na, plane0 + H + (long long)pNX * H + S2 * index, e * pNX, e * na, nb,
@@ -112,7 +113,7 @@ struct SolverBase {
     return CGFDM_OK;
   }
   int check_plane(int comp, int axis, int index) {
-    const int n = axis == 0 ? pNX : axis == 1 ? pNY : pNZ;
+    const int n = axis == 0 ? pXe : axis == 1 ? pNY : pNZ;
     if (comp < 0 || comp > 8 || axis < 0 || axis > 2 || index < 3 || index >= n - 3)
       return fail(CGFDM_ERR_ARG, "slice component, axis or index out of range");
     return CGFDM_OK;
@@ -136,6 +137,20 @@ struct SolverBase {
     *p = (X*)q;
     return CGFDM_OK;
   }
+  // Grid-shaped buffers of the stage kernels in the row-aligned layout:
+  // allocated with spare elements and offset so that the first interior
+  // column (i = 3) of every row sits on a 128-byte boundary (with a row pitch
+  // that is a multiple of 32 elements). `galign` = false keeps offset 0.
+  bool galign = true;
+  template <class X>
+  int galloc(X** p, long long count) {
+    X* q = nullptr;
+    const long long line = 128 / (long long)sizeof(X);
+    if (int rc = dalloc(&q, count + 2 * line)) return rc;
+    *p = q + (galign ? (line - 3) % line : 0);
+    return CGFDM_OK;
+  }
+
   void dfree(void* p) {
     for (size_t q = 0; q < allocs.size(); ++q)
       if (allocs[q] == p) {
@@ -221,8 +236,13 @@ float to_float<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
 template <class R, class TM>
 struct Solver : SolverBase {
   cgfdm_desc d{};
+  // device layout: row pitch NX (nx + 6 rounded up to 32 elements in the
+  // row-aligned layout), plane S = NX NY, volume N = S NZ; the host-facing
+  // arrays of the C ABI use the tight layout tNX = nx + 6 (tS, tN)
   int NX = 0, NY = 0, NZ = 0;
   long long N = 0, S = 0;
+  int tNX = 0;
+  long long tS = 0, tN = 0;
   double Z = 1.0;  // stress carried as T / Z
   R *A = nullptr, *B = nullptr, *C = nullptr, *T = nullptr;
   TM* coef = nullptr;
@@ -279,14 +299,17 @@ struct Solver : SolverBase {
       if (v >= -1 && v <= 4096) kc = v;
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
-    NX = d.nx + 2 * cg::H;
+    if (const char* e = std::getenv("CGFDM_ROW_ALIGN")) galign = std::atoi(e) != 0;
+    // setup runs on the tight layout; its products are repacked into the
+    // row-aligned layout below
+    tNX = d.nx + 2 * cg::H;
+    NX = tNX;
     NY = d.ny + 2 * cg::H;
     NZ = d.nz + 2 * cg::H;
     S = (long long)NX * NY;
     N = S * NZ;
-    pNX = NX;
-    pNY = NY;
-    pNZ = NZ;
+    tS = S;
+    tN = N;
     int rc;
     // --- temporaries: coordinates, medium, FP64 metrics
     float *dcoord = nullptr, *dmed = nullptr;
@@ -309,23 +332,45 @@ struct Solver : SolverBase {
     }
     if (d.coef_storage != 0) Z = std::sqrt(ranges[6] * ranges[7]) * std::sqrt(ranges[4] * ranges[5]);
 
-    // --- packed coefficients
-    if ((rc = dalloc(&coef, (long long)cg::M_NCOEF * N))) return rc;
-    if ((rc = dalloc(&jac, N))) return rc;
-    cg::launch_pack<R, TM>(met, dmed, coef, jac, N, Z, stream);
-    if ((rc = dalloc(&rdzdx, 9 * S))) return rc;
-    if ((rc = dalloc(&rdzdy, 9 * S))) return rc;
-    if (d.free_surface) cg::launch_rdz<R>(met, dmed, rdzdx, rdzdy, d.nx, d.ny, d.nz, NX, NY, N, stream);
+    // --- packed coefficients (tight), then the row-aligned copies
+    TM* coef_t = nullptr;
+    R *jac_t = nullptr, *rdzdx_t = nullptr, *rdzdy_t = nullptr;
+    if ((rc = dalloc(&coef_t, (long long)cg::M_NCOEF * N))) return rc;
+    if ((rc = dalloc(&jac_t, N))) return rc;
+    cg::launch_pack<R, TM>(met, dmed, coef_t, jac_t, N, Z, stream);
+    if ((rc = dalloc(&rdzdx_t, 9 * S))) return rc;
+    if ((rc = dalloc(&rdzdy_t, 9 * S))) return rc;
+    if (d.free_surface) cg::launch_rdz<R>(met, dmed, rdzdx_t, rdzdy_t, d.nx, d.ny, d.nz, NX, NY, N, stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));
     dfree(met);
     dfree(dmed);
     dfree(dcoord);
+    NX = galign ? (tNX + 31) / 32 * 32 : tNX;
+    S = (long long)NX * NY;
+    N = S * NZ;
+    pNX = NX;
+    pNY = NY;
+    pNZ = NZ;
+    pXe = tNX;
+    if ((rc = galloc(&coef, (long long)cg::M_NCOEF * N))) return rc;
+    if ((rc = galloc(&jac, N))) return rc;
+    if ((rc = galloc(&rdzdx, 9 * S))) return rc;
+    if ((rc = galloc(&rdzdy, 9 * S))) return rc;
+    if ((rc = repack(coef, coef_t, (long long)cg::M_NCOEF * NY * NZ, true))) return rc;
+    if ((rc = repack(jac, jac_t, (long long)NY * NZ, true))) return rc;
+    if ((rc = repack(rdzdx, rdzdx_t, 9LL * NY, true))) return rc;
+    if ((rc = repack(rdzdy, rdzdy_t, 9LL * NY, true))) return rc;
+    HIPCHK(hipStreamSynchronize(stream));
+    dfree(coef_t);
+    dfree(jac_t);
+    dfree(rdzdx_t);
+    dfree(rdzdy_t);
 
     // --- PML profiles (init_pml_para.cpp:97-239), host FP64, then upload
     {
-      int nn[3] = {NX, NY, NZ};
-      std::vector<R> hp(3 * (size_t)(NX + NY + NZ));
+      int nn[3] = {tNX, NY, NZ};
+      std::vector<R> hp(3 * (size_t)(tNX + NY + NZ));
       double L = d.dh * (d.npml - 1);
       double Rc = d.npml > 0 ? std::pow(10.0, -(std::log10((double)d.npml) - 1.0) / std::log10(2.0) - 3.0) : 1.0;
       double d0 = d.npml > 1 ? -(2.0 + 1.0) * 6000.0 / (2.0 * L) * std::log(Rc) : 0.0;
@@ -386,25 +431,58 @@ struct Solver : SolverBase {
       }
     }
     // --- wavefield buffers and observers
-    if ((rc = dalloc(&A, 9 * N))) return rc;
-    if ((rc = dalloc(&B, 9 * N))) return rc;
-    if ((rc = dalloc(&C, 9 * N))) return rc;
-    if ((rc = dalloc(&T, 9 * N))) return rc;
+    if ((rc = galloc(&A, 9 * N))) return rc;
+    if ((rc = galloc(&B, 9 * N))) return rc;
+    if ((rc = galloc(&C, 9 * N))) return rc;
+    if ((rc = galloc(&T, 9 * N))) return rc;
     if ((rc = dalloc(&pgv, 2LL * d.nx * d.ny))) return rc;
     if ((rc = dalloc(&dis, 3 * S))) return rc;
     HIPCHK(hipStreamSynchronize(stream));
     return CGFDM_OK;
   }
 
+  // rows of tNX elements between the tight layout (pitch tNX) and the device
+  // layout (pitch NX): `to_dev` copies tight -> device, else device -> tight;
+  // both pointers are device pointers (hipMemcpy2D, no kernel)
+  template <class X>
+  int repack(X* dev, const X* tight, long long rows, bool to_dev) {
+    const size_t e = sizeof(X);
+    if (to_dev)
+      HIPCHK(hipMemcpy2DAsync(dev, e * NX, tight, e * tNX, e * tNX, (size_t)rows, hipMemcpyDeviceToDevice, stream));
+    else
+      HIPCHK(hipMemcpy2DAsync((void*)tight, e * tNX, dev, e * NX, e * tNX, (size_t)rows, hipMemcpyDeviceToDevice,
+                              stream));
+    return CGFDM_OK;
+  }
+  // `planes` device volumes (pitch NX) to / from a tight host array
+  template <class X>
+  int to_host(const X* dev, X* host, int planes) {
+    const size_t e = sizeof(X);
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy2D(host, e * tNX, dev, e * NX, e * tNX, (size_t)planes * NY * NZ, hipMemcpyDeviceToHost));
+    return CGFDM_OK;
+  }
+  template <class X>
+  int to_dev(X* dev, const X* host, int planes) {
+    const size_t e = sizeof(X);
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy2D(dev, e * NX, host, e * tNX, e * tNX, (size_t)planes * NY * NZ, hipMemcpyHostToDevice));
+    return CGFDM_OK;
+  }
+  // tight linear cell index -> device cell index (-1 outside the interior)
+  long long dev_cell(long long c) const {
+    if (c < 0 || c >= tN) return -1;
+    const int i = (int)(c % tNX), j = (int)((c / tNX) % NY), k = (int)(c / tS);
+    if (i < cg::H || i >= tNX - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H || k >= NZ - cg::H) return -1;
+    return (long long)i + (long long)NX * j + (long long)k * S;
+  }
+
   int set_source(int np, int nt, double dt, const long long* cell, const float* rate) override {
     if (np < 0 || (np > 0 && (nt < 2 || dt <= 0 || !cell || !rate))) return fail(CGFDM_ERR_SOURCE, "bad source table");
     std::map<long long, std::vector<double>> acc;
     for (int p = 0; p < np; ++p) {
-      long long c = cell[p];
-      int i = (int)(c % NX), j = (int)((c / NX) % NY), k = (int)(c / S);
-      if (c < 0 || c >= N || i < cg::H || i >= NX - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H ||
-          k >= NZ - cg::H)
-        return fail(CGFDM_ERR_SOURCE, "source cell outside the interior");
+      const long long c = dev_cell(cell[p]);
+      if (c < 0) return fail(CGFDM_ERR_SOURCE, "source cell outside the interior");
       auto& v = acc[c];
       if (v.empty()) v.assign((size_t)nt * 6, 0.0);
       const float* r = rate + (size_t)p * nt * 6;
@@ -448,7 +526,7 @@ struct Solver : SolverBase {
       ++Jp;
     }
     src_of_point.resize(np);
-    for (int p = 0; p < np; ++p) src_of_point[p] = merged[cell[p]];
+    for (int p = 0; p < np; ++p) src_of_point[p] = merged[dev_cell(cell[p])];
     if ((rc = dalloc(&src_cell, src_np))) return rc;
     if ((rc = dalloc(&src_rate, (long long)hr.size()))) return rc;
     HIPCHK(hipMemcpy(src_cell, hc.data(), sizeof(long long) * src_np, hipMemcpyHostToDevice));
@@ -461,7 +539,7 @@ struct Solver : SolverBase {
     std::vector<long long> hc(n);
     for (int s = 0; s < n; ++s) {
       int i = ijk[3 * s], j = ijk[3 * s + 1], k = ijk[3 * s + 2];
-      if (i < cg::H || i >= NX - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H || k >= NZ - cg::H)
+      if (i < cg::H || i >= pXe - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H || k >= NZ - cg::H)
         return fail(CGFDM_ERR_ARG, "station outside the interior");
       hc[s] = (long long)i + (long long)NX * j + (long long)k * S;
     }
@@ -585,11 +663,10 @@ struct Solver : SolverBase {
 
   int get_field(int comp, double* out) override {
     if (comp < 0 || comp > 8) return fail(CGFDM_ERR_ARG, "component out of range");
-    HIPCHK(hipStreamSynchronize(stream));
-    std::vector<R> h(N);
-    HIPCHK(hipMemcpy(h.data(), A + (size_t)comp * N, sizeof(R) * N, hipMemcpyDeviceToHost));
+    std::vector<R> h(tN);
+    if (int rc = to_host(A + (size_t)comp * N, h.data(), 1)) return rc;
     double sc = comp >= 3 ? Z : 1.0;
-    for (long long p = 0; p < N; ++p) out[p] = (double)h[p] * sc;
+    for (long long p = 0; p < tN; ++p) out[p] = (double)h[p] * sc;
     return CGFDM_OK;
   }
 
@@ -606,42 +683,41 @@ struct Solver : SolverBase {
   int set_field(int comp, const double* in) override {
     if (comp < 0 || comp > 8) return fail(CGFDM_ERR_ARG, "component out of range");
     HIPCHK(hipStreamSynchronize(stream));
-    std::vector<R> h(N);
+    std::vector<R> h(tN);
     double sc = comp >= 3 ? 1.0 / Z : 1.0;
-    for (long long p = 0; p < N; ++p) h[p] = R(in[p] * sc);
-    HIPCHK(hipMemcpy(A + (size_t)comp * N, h.data(), sizeof(R) * N, hipMemcpyHostToDevice));
-    return CGFDM_OK;
+    for (long long p = 0; p < tN; ++p) h[p] = R(in[p] * sc);
+    return to_dev(A + (size_t)comp * N, h.data(), 1);
   }
 
   int get_setup(int what, double* out) override {
     HIPCHK(hipStreamSynchronize(stream));
     if (what >= 0 && what <= 12 && what != 9) {
       int plane = what < 9 ? what : what - 1;  // 10,11,12 -> M_MU.. (9,10,11)
-      std::vector<TM> h(N);
-      HIPCHK(hipMemcpy(h.data(), coef + (size_t)plane * N, sizeof(TM) * N, hipMemcpyDeviceToHost));
+      std::vector<TM> h(tN);
+      if (int rc = to_host(coef + (size_t)plane * N, h.data(), 1)) return rc;
       double sc = what == 10 || what == 11 ? Z : what == 12 ? 1.0 / Z : 1.0;
-      for (long long p = 0; p < N; ++p) {
+      for (long long p = 0; p < tN; ++p) {
         double v = sizeof(TM) == 8 ? (double)*(const double*)(const void*)&h[p] : (double)to_float(h[p]);
         out[p] = v * sc;
       }
       return CGFDM_OK;
     }
     if (what == 9) {
-      std::vector<R> h(N);
-      HIPCHK(hipMemcpy(h.data(), jac, sizeof(R) * N, hipMemcpyDeviceToHost));
-      for (long long p = 0; p < N; ++p) out[p] = (double)h[p];
+      std::vector<R> h(tN);
+      if (int rc = to_host(jac, h.data(), 1)) return rc;
+      for (long long p = 0; p < tN; ++p) out[p] = (double)h[p];
       return CGFDM_OK;
     }
     if ((what >= 20 && what < 29) || (what >= 30 && what < 39)) {
       const R* src = (what < 30 ? rdzdx : rdzdy) + (size_t)(what % 10) * S;
-      std::vector<R> h(S);
-      HIPCHK(hipMemcpy(h.data(), src, sizeof(R) * S, hipMemcpyDeviceToHost));
-      for (long long p = 0; p < S; ++p) out[p] = (double)h[p];
+      std::vector<R> h(tS);
+      HIPCHK(hipMemcpy2D(h.data(), sizeof(R) * tNX, src, sizeof(R) * NX, sizeof(R) * tNX, NY, hipMemcpyDeviceToHost));
+      for (long long p = 0; p < tS; ++p) out[p] = (double)h[p];
       return CGFDM_OK;
     }
     if (what >= 40 && what < 49) {
       int ax = (what - 40) / 3, kind = (what - 40) % 3;
-      int n = ax == 0 ? NX : ax == 1 ? NY : NZ;
+      int n = ax == 0 ? tNX : ax == 1 ? NY : NZ;
       const R* src = kind == 0 ? pml.alpha[ax] : kind == 1 ? pml.beta[ax] : pml.d[ax];
       std::vector<R> h(n);
       HIPCHK(hipMemcpy(h.data(), src, sizeof(R) * n, hipMemcpyDeviceToHost));
@@ -900,9 +976,14 @@ struct Solver : SolverBase {
           const size_t e = ((size_t)t * src_np + p) * 6 + q;
           hg[hcell[p]] += hs[e] * src_raw[e] * src_vol[p];
         }
-    if (gmu) std::memcpy(gmu, hg.data(), sizeof(double) * N);
-    if (glam) std::memcpy(glam, hg.data() + N, sizeof(double) * N);
-    if (gb) std::memcpy(gb, hg.data() + 2 * N, sizeof(double) * N);
+    // device layout -> tight layout of the C ABI
+    double* outs[3] = {gmu, glam, gb};
+    for (int g = 0; g < 3; ++g) {
+      if (!outs[g]) continue;
+      const double* src = hg.data() + (size_t)g * N;
+      for (long long r = 0; r < (long long)NY * NZ; ++r)
+        std::memcpy(outs[g] + r * tNX, src + r * NX, sizeof(double) * tNX);
+    }
     if (grate)
       for (size_t pi = 0; pi < src_of_point.size(); ++pi) {
         const int p = src_of_point[pi];
@@ -968,6 +1049,7 @@ struct AwenoSolver : SolverBase {
     pNX = NX;
     pNY = NY;
     pNZ = NZ;
+    pXe = NX;
     int rc;
     float *dcoord = nullptr, *dmed = nullptr;
     double* cov = nullptr;
@@ -1048,7 +1130,7 @@ struct AwenoSolver : SolverBase {
     std::vector<long long> hc(n);
     for (int s = 0; s < n; ++s) {
       int i = ijk[3 * s], j = ijk[3 * s + 1], k = ijk[3 * s + 2];
-      if (i < cg::H || i >= NX - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H || k >= NZ - cg::H)
+      if (i < cg::H || i >= pXe - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H || k >= NZ - cg::H)
         return fail(CGFDM_ERR_ARG, "station outside the interior");
       hc[s] = (long long)i + (long long)NX * j + (long long)k * S;
     }

@@ -102,12 +102,36 @@ __device__ __forceinline__ void pml_face(const FaceBuf<R>& F, long long pos, con
 // (differences from global memory) and the z-marching kernel (x/y from an LDS
 // plane, z from a register window).
 // ---------------------------------------------------------------------------
+// Streaming accesses of a stage (coefficients, RK base and accumulator, the
+// two outputs) are used once per cell; with CGFDM_NT_STREAMS they carry the
+// non-temporal hint so that they do not displace the stencil rows of W in L2.
+#ifdef CGFDM_NT_STREAMS
+template <class T>
+__device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
+template <class T>
+__device__ __forceinline__ void st_stream(T* p, T v) { __builtin_nontemporal_store(v, p); }
+#else
+template <class T>
+__device__ __forceinline__ T ld_stream(const T* p) { return *p; }
+template <class T>
+__device__ __forceinline__ void st_stream(T* p, T v) { *p = v; }
+#endif
+template <class T>
+__device__ __forceinline__ float ld_coef(const T* p) { return Store<T>::ld(p); }
+#ifdef CGFDM_NT_STREAMS
+template <>
+__device__ __forceinline__ float ld_coef<float>(const float* p) { return __builtin_nontemporal_load(p); }
+#endif
+
 template <class R, class TM, int FB1, int FB2, int FB3, int STAGE, class XD, class YD, class ZD, class WC>
 __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __restrict__ coef, int i, int j, int k,
                                           XD xd, YD yd, ZD zd, WC wc) {
   const long long N = a.N, S = a.S;
   const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
-  auto cf = [&](int c) -> R { return R(Store<TM>::ld(coef + (long long)c * N + p)); };
+  auto cf = [&](int c) -> R {
+    if constexpr (sizeof(TM) == 4) return R(ld_coef(coef + (long long)c * N + p));
+    else return R(Store<TM>::ld(coef + (long long)c * N + p));
+  };
   const R mu = cf(M_MU), lam = cf(M_LAM), bu = cf(M_B);
   const PmlArgs<R>& P = a.pml;
   R h[9], D[9], Q[9];
@@ -176,16 +200,16 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
     const long long q = (long long)v * N + p;
     if constexpr (STAGE == 0) {
       const R m = wc(v);
-      a.wt[q] = m + R(RK_B1) * h[v];
-      a.wout[q] = m + R(RK_A2) * h[v];
+      st_stream(a.wt + q, R(m + R(RK_B1) * h[v]));
+      st_stream(a.wout + q, R(m + R(RK_A2) * h[v]));
     } else if constexpr (STAGE == 1) {
-      a.wt[q] = a.wt[q] + R(RK_B2) * h[v];
-      a.wout[q] = a.wm[q] + R(RK_A3) * h[v];
+      st_stream(a.wt + q, R(ld_stream(a.wt + q) + R(RK_B2) * h[v]));
+      st_stream(a.wout + q, R(ld_stream(a.wm + q) + R(RK_A3) * h[v]));
     } else if constexpr (STAGE == 2) {
-      a.wt[q] = a.wt[q] + R(RK_B3) * h[v];
-      a.wout[q] = a.wm[q] + h[v];
+      st_stream(a.wt + q, R(ld_stream(a.wt + q) + R(RK_B3) * h[v]));
+      st_stream(a.wout + q, R(ld_stream(a.wm + q) + h[v]));
     } else {
-      a.wout[q] = a.wt[q] + R(RK_B4) * h[v];
+      st_stream(a.wout + q, R(ld_stream(a.wt + q) + R(RK_B4) * h[v]));
     }
   }
 }

@@ -76,6 +76,10 @@ hipError_t hipMemcpy2D(void* d, size_t dp, const void* s, size_t sp, size_t w, s
   for (size_t r = 0; r < h; ++r) memmove((char*)d + r * dp, (const char*)s + r * sp, w);
   return 0;
 }
+hipError_t hipMemcpy2DAsync(void* d, size_t dp, const void* s, size_t sp, size_t w, size_t h, int kind, hipStream_t st) {
+  (void)st;
+  return hipMemcpy2D(d, dp, s, sp, w, h, kind);
+}
 hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t st) { (void)st; memset(d, v, n); return 0; }
 hipError_t hipStreamSynchronize(hipStream_t st) { (void)st; return g_sticky; }
 hipError_t hipGetLastError(void) { hipError_t e = g_last; g_last = 0; return e; }

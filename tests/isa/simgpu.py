@@ -23,7 +23,10 @@ import gcnemu as G  # noqa: E402
 REPO = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(REPO, "cgfdm3d-eqr-sc_amd", "csrc")
 BUILD = os.path.join(HERE, "_build")
-ASMDIR = os.path.join(BUILD, "asm")
+# extra preprocessor definitions of the kernel build under test (space separated,
+# e.g. CGFDM_ISA_DEFINES="CGFDM_NT_STREAMS"), compiled into their own folder
+DEFINES = os.environ.get("CGFDM_ISA_DEFINES", "").split()
+ASMDIR = os.path.join(BUILD, "asm" + "".join("_" + d for d in DEFINES))
 FAKEHIP = os.path.join(BUILD, "libamdhip64.so.7")
 
 
@@ -51,7 +54,7 @@ def build_asm(force=False):
             continue
         cmd = ["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature",
                "-Xclang", "-packed-fp32-ops", "-I", CSRC, "-I", os.path.join(REPO, "include"), "--cuda-device-only",
-               "-S", src, "-o", out]
+               "-S", src, "-o", out] + ["-D" + d for d in DEFINES]
         procs.append(subprocess.Popen(cmd, stderr=subprocess.DEVNULL))
     for p in procs:
         assert p.wait() == 0
