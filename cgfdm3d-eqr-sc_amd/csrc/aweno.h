@@ -11,7 +11,8 @@ enum { C_J = 9, C_MU = 10, C_LAM = 11, C_B = 12, C_TXI = 13, C_TET = 16, C_N = 1
 
 template <class R>
 struct StageArgs {
-  int NX, NY, NZ;
+  int NX, NY, NZ;      // NX: device row pitch
+  int XE;              // padded x extent nx + 6 (NX >= XE in the row-aligned layout)
   long long N, S;
   int stage;           // TVD-RK3 stage 0..2
   R DT, rDH, alpha;    // alpha: Lax-Friedrichs speed (global Vp max)

@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(256) aw_faces(StageArgs<R> a) {
   const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
   const int k = 2 + blockIdx.z;
-  if (i >= a.NX - 3 || j >= a.NY - 3) return;
+  if (i >= a.XE - 3 || j >= a.NY - 3) return;
   const long long p = (long long)i + (long long)a.NX * j + (long long)k * a.S;
   const bool ii = i >= H, ji = j >= H, ki = k >= H;
   if (ji && ki) face_flux<R>(a, p, 1, 0);
@@ -141,10 +141,10 @@ __global__ void __launch_bounds__(256) aw_update(StageArgs<R> a) {
   const int i = blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
   const int k = blockIdx.z;
-  if (i >= a.NX || j >= a.NY) return;
+  if (i >= a.XE || j >= a.NY) return;
   const long long N = a.N;
   const long long p = (long long)i + (long long)a.NX * j + (long long)k * a.S;
-  const bool interior = i >= H && i < a.NX - H && j >= H && j < a.NY - H && k >= H && k < a.NZ - H;
+  const bool interior = i >= H && i < a.XE - H && j >= H && j < a.NY - H && k >= H && k < a.NZ - H;
   const int ktop3 = a.NZ - H - 3;
   if (interior && k >= ktop3) return;
   R h[9];
@@ -439,13 +439,13 @@ __global__ void aw_record(const R* __restrict__ W, const R* __restrict__ cjm, lo
 
 template <class R>
 void launch_faces(const StageArgs<R>& a, hipStream_t st) {
-  const dim3 grid((a.NX - 5 + 63) / 64, (a.NY - 5 + 3) / 4, a.NZ - 5), block(256);
+  const dim3 grid((a.XE - 5 + 63) / 64, (a.NY - 5 + 3) / 4, a.NZ - 5), block(256);
   aw_faces<R><<<grid, block, 0, st>>>(a);
 }
 
 template <class R>
 void launch_update(const StageArgs<R>& a, int nx, int ny, hipStream_t st) {
-  const dim3 grid((a.NX + 63) / 64, (a.NY + 3) / 4, a.NZ), block(256);
+  const dim3 grid((a.XE + 63) / 64, (a.NY + 3) / 4, a.NZ), block(256);
   aw_update<R><<<grid, block, 0, st>>>(a);
   const dim3 gtop((nx + 63) / 64, (ny + 3) / 4, 3);
   aw_update_top<R><<<gtop, block, 0, st>>>(a, nx, ny);

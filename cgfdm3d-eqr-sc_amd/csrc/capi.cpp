@@ -112,6 +112,43 @@ struct SolverBase {
     }
     return CGFDM_OK;
   }
+  // Rows of pXe elements between the tight layout of the C ABI (pitch pXe)
+  // and the device layout (pitch pNX); strided hipMemcpy2D, no kernel.
+  // repack: device tight <-> device pitched (`to_dev` = tight -> pitched)
+  template <class X>
+  int repack(X* dev, const X* tight, long long rows, bool to_dev) {
+    const size_t e = sizeof(X);
+    if (to_dev)
+      HIPCHK(hipMemcpy2DAsync(dev, e * pNX, tight, e * pXe, e * pXe, (size_t)rows, hipMemcpyDeviceToDevice, stream));
+    else
+      HIPCHK(hipMemcpy2DAsync((void*)tight, e * pXe, dev, e * pNX, e * pXe, (size_t)rows, hipMemcpyDeviceToDevice,
+                              stream));
+    return CGFDM_OK;
+  }
+  // `planes` device volumes to / from a tight host array
+  template <class X>
+  int to_host(const X* dev, X* host, int planes) {
+    const size_t e = sizeof(X);
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy2D(host, e * pXe, dev, e * pNX, e * pXe, (size_t)planes * pNY * pNZ, hipMemcpyDeviceToHost));
+    return CGFDM_OK;
+  }
+  template <class X>
+  int to_dev(X* dev, const X* host, int planes) {
+    const size_t e = sizeof(X);
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy2D(dev, e * pNX, host, e * pXe, e * pXe, (size_t)planes * pNY * pNZ, hipMemcpyHostToDevice));
+    return CGFDM_OK;
+  }
+  // tight linear cell index -> device cell index (-1 outside the interior)
+  long long dev_cell(long long c) const {
+    const long long tS = (long long)pXe * pNY;
+    if (c < 0 || c >= tS * pNZ) return -1;
+    const int i = (int)(c % pXe), j = (int)((c / pXe) % pNY), k = (int)(c / tS);
+    if (i < 3 || i >= pXe - 3 || j < 3 || j >= pNY - 3 || k < 3 || k >= pNZ - 3) return -1;
+    return (long long)i + (long long)pNX * j + (long long)k * pNX * pNY;
+  }
+
   int check_plane(int comp, int axis, int index) {
     const int n = axis == 0 ? pXe : axis == 1 ? pNY : pNZ;
     if (comp < 0 || comp > 8 || axis < 0 || axis > 2 || index < 3 || index >= n - 3)
@@ -439,42 +476,6 @@ struct Solver : SolverBase {
     if ((rc = dalloc(&dis, 3 * S))) return rc;
     HIPCHK(hipStreamSynchronize(stream));
     return CGFDM_OK;
-  }
-
-  // rows of tNX elements between the tight layout (pitch tNX) and the device
-  // layout (pitch NX): `to_dev` copies tight -> device, else device -> tight;
-  // both pointers are device pointers (hipMemcpy2D, no kernel)
-  template <class X>
-  int repack(X* dev, const X* tight, long long rows, bool to_dev) {
-    const size_t e = sizeof(X);
-    if (to_dev)
-      HIPCHK(hipMemcpy2DAsync(dev, e * NX, tight, e * tNX, e * tNX, (size_t)rows, hipMemcpyDeviceToDevice, stream));
-    else
-      HIPCHK(hipMemcpy2DAsync((void*)tight, e * tNX, dev, e * NX, e * tNX, (size_t)rows, hipMemcpyDeviceToDevice,
-                              stream));
-    return CGFDM_OK;
-  }
-  // `planes` device volumes (pitch NX) to / from a tight host array
-  template <class X>
-  int to_host(const X* dev, X* host, int planes) {
-    const size_t e = sizeof(X);
-    HIPCHK(hipStreamSynchronize(stream));
-    HIPCHK(hipMemcpy2D(host, e * tNX, dev, e * NX, e * tNX, (size_t)planes * NY * NZ, hipMemcpyDeviceToHost));
-    return CGFDM_OK;
-  }
-  template <class X>
-  int to_dev(X* dev, const X* host, int planes) {
-    const size_t e = sizeof(X);
-    HIPCHK(hipStreamSynchronize(stream));
-    HIPCHK(hipMemcpy2D(dev, e * NX, host, e * tNX, e * tNX, (size_t)planes * NY * NZ, hipMemcpyHostToDevice));
-    return CGFDM_OK;
-  }
-  // tight linear cell index -> device cell index (-1 outside the interior)
-  long long dev_cell(long long c) const {
-    if (c < 0 || c >= tN) return -1;
-    const int i = (int)(c % tNX), j = (int)((c / tNX) % NY), k = (int)(c / tS);
-    if (i < cg::H || i >= tNX - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H || k >= NZ - cg::H) return -1;
-    return (long long)i + (long long)NX * j + (long long)k * S;
   }
 
   int set_source(int np, int nt, double dt, const long long* cell, const float* rate) override {
@@ -1003,8 +1004,12 @@ struct Solver : SolverBase {
 template <class R>
 struct AwenoSolver : SolverBase {
   cgfdm_desc d{};
+  // device layout (row pitch NX) and the tight layout of the C ABI (tNX), as
+  // in the CGFDM solver
   int NX = 0, NY = 0, NZ = 0;
   long long N = 0, S = 0;
+  int tNX = 0;
+  long long tS = 0, tN = 0;
   R* buf[3] = {nullptr, nullptr, nullptr};
   int cur = 0;
   R* cjm = nullptr;
@@ -1041,15 +1046,15 @@ struct AwenoSolver : SolverBase {
       if (v >= -1 && v <= 4096) kc = v;
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
-    NX = d.nx + 2 * cg::H;
+    if (const char* e = std::getenv("CGFDM_ROW_ALIGN")) galign = std::atoi(e) != 0;
+    tNX = d.nx + 2 * cg::H;
+    NX = tNX;
     NY = d.ny + 2 * cg::H;
     NZ = d.nz + 2 * cg::H;
     S = (long long)NX * NY;
     N = S * NZ;
-    pNX = NX;
-    pNY = NY;
-    pNZ = NZ;
-    pXe = NX;
+    tS = S;
+    tN = N;
     int rc;
     float *dcoord = nullptr, *dmed = nullptr;
     double* cov = nullptr;
@@ -1064,18 +1069,31 @@ struct AwenoSolver : SolverBase {
       return fail(CGFDM_ERR_CFL, "DT " + std::to_string(d.dt) + " exceeds the CFL limit " + std::to_string(dtmax));
     alpha = R((float)ranges[5]);  // vp_max_for_SCFDM (calc_CFL.cpp:317-321)
     cg::launch_cov_cmm(dcoord, cov, NX, NY, NZ, N, d.dh, stream);
-    if ((rc = dalloc(&cjm, (long long)aw::C_N * N))) return rc;
-    cg::launch_pack_cmm<R>(cov, dmed, cjm, NX, NY, NZ, N, stream);
+    R* cjm_t = nullptr;
+    if ((rc = dalloc(&cjm_t, (long long)aw::C_N * N))) return rc;
+    cg::launch_pack_cmm<R>(cov, dmed, cjm_t, NX, NY, NZ, N, stream);
     HIPCHK(hipGetLastError());
-    jac_host.assign(N, 0.0);
+    jac_host.assign(N, 0.0);  // tight layout
     HIPCHK(hipMemcpyAsync(jac_host.data(), cov + 9 * N, sizeof(double) * N, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     dfree(cov);
     dfree(dmed);
     dfree(dcoord);
+    // row-aligned device layout (see the CGFDM solver)
+    NX = galign ? (tNX + 31) / 32 * 32 : tNX;
+    S = (long long)NX * NY;
+    N = S * NZ;
+    pNX = NX;
+    pNY = NY;
+    pNZ = NZ;
+    pXe = tNX;
+    if ((rc = galloc(&cjm, (long long)aw::C_N * N))) return rc;
+    if ((rc = repack(cjm, cjm_t, (long long)aw::C_N * NY * NZ, true))) return rc;
+    HIPCHK(hipStreamSynchronize(stream));
+    dfree(cjm_t);
     for (int q = 0; q < 3; ++q)
-      if ((rc = dalloc(&buf[q], 9 * N))) return rc;
-    if ((rc = dalloc(&F, 27 * N))) return rc;
+      if ((rc = galloc(&buf[q], 9 * N))) return rc;
+    if ((rc = galloc(&F, 27 * N))) return rc;
     if ((rc = dalloc(&pgv, 2LL * d.nx * d.ny))) return rc;
     if ((rc = dalloc(&dis, 3 * S))) return rc;
     HIPCHK(hipStreamSynchronize(stream));
@@ -1086,11 +1104,8 @@ struct AwenoSolver : SolverBase {
     if (np < 0 || (np > 0 && (nt < 2 || dt <= 0 || !cell || !rate))) return fail(CGFDM_ERR_SOURCE, "bad source table");
     std::map<long long, std::vector<double>> acc;
     for (int p = 0; p < np; ++p) {
-      long long c = cell[p];
-      int i = (int)(c % NX), j = (int)((c / NX) % NY), k = (int)(c / S);
-      if (c < 0 || c >= N || i < cg::H || i >= NX - cg::H || j < cg::H || j >= NY - cg::H || k < cg::H ||
-          k >= NZ - cg::H)
-        return fail(CGFDM_ERR_SOURCE, "source cell outside the interior");
+      const long long c = cell[p];  // tight index; the device index follows below
+      if (dev_cell(c) < 0) return fail(CGFDM_ERR_SOURCE, "source cell outside the interior");
       auto& v = acc[c];
       if (v.empty()) v.assign((size_t)nt * 6, 0.0);
       const float* r = rate + (size_t)p * nt * 6;
@@ -1109,11 +1124,11 @@ struct AwenoSolver : SolverBase {
     std::vector<R> hr((size_t)src_np * nt * 6);
     int Jp = 0;
     for (auto& kv : acc) {
-      long long c = kv.first;
+      const long long ct = kv.first, c = dev_cell(ct);
       hc.push_back(c);
       R mv;
       HIPCHK(hipMemcpy(&mv, cjm + (long long)aw::C_MU * N + c, sizeof(R), hipMemcpyDeviceToHost));
-      double sc = (double)mv / (jac_host[c] * d.dh * d.dh * d.dh);  // mu/(J DH^3) (addMoment.cpp:27-76)
+      double sc = (double)mv / (jac_host[ct] * d.dh * d.dh * d.dh);  // mu/(J DH^3) (addMoment.cpp:27-76)
       for (int t = 0; t < nt; ++t)
         for (int q = 0; q < 6; ++q) hr[((size_t)t * src_np + Jp) * 6 + q] = R(kv.second[(size_t)t * 6 + q] * sc);
       ++Jp;
@@ -1171,6 +1186,7 @@ struct AwenoSolver : SolverBase {
       for (int irk = 0; irk < 3; ++irk) {
         aw::StageArgs<R> a{};
         a.NX = NX;
+        a.XE = tNX;
         a.NY = NY;
         a.NZ = NZ;
         a.N = N;
@@ -1237,10 +1253,9 @@ struct AwenoSolver : SolverBase {
 
   int get_field(int comp, double* out) override {
     if (comp < 0 || comp > 8) return fail(CGFDM_ERR_ARG, "component out of range");
-    HIPCHK(hipStreamSynchronize(stream));
-    std::vector<R> h(N);
-    HIPCHK(hipMemcpy(h.data(), buf[cur] + (size_t)comp * N, sizeof(R) * N, hipMemcpyDeviceToHost));
-    for (long long p = 0; p < N; ++p) out[p] = (double)h[p];
+    std::vector<R> h(tN);
+    if (int rc = to_host(buf[cur] + (size_t)comp * N, h.data(), 1)) return rc;
+    for (long long p = 0; p < tN; ++p) out[p] = (double)h[p];
     return CGFDM_OK;
   }
 
@@ -1283,18 +1298,16 @@ struct AwenoSolver : SolverBase {
   int set_field(int comp, const double* in) override {
     if (comp < 0 || comp > 8) return fail(CGFDM_ERR_ARG, "component out of range");
     HIPCHK(hipStreamSynchronize(stream));
-    std::vector<R> h(N);
-    for (long long p = 0; p < N; ++p) h[p] = R(in[p]);
-    HIPCHK(hipMemcpy(buf[cur] + (size_t)comp * N, h.data(), sizeof(R) * N, hipMemcpyHostToDevice));
-    return CGFDM_OK;
+    std::vector<R> h(tN);
+    for (long long p = 0; p < tN; ++p) h[p] = R(in[p]);
+    return to_dev(buf[cur] + (size_t)comp * N, h.data(), 1);
   }
 
   int get_setup(int what, double* out) override {
     if (what < 0 || what >= aw::C_N) return fail(CGFDM_ERR_ARG, "unknown setup product");
-    HIPCHK(hipStreamSynchronize(stream));
-    std::vector<R> h(N);
-    HIPCHK(hipMemcpy(h.data(), cjm + (size_t)what * N, sizeof(R) * N, hipMemcpyDeviceToHost));
-    for (long long p = 0; p < N; ++p) out[p] = (double)h[p];
+    std::vector<R> h(tN);
+    if (int rc = to_host(cjm + (size_t)what * N, h.data(), 1)) return rc;
+    for (long long p = 0; p < tN; ++p) out[p] = (double)h[p];
     return CGFDM_OK;
   }
 };
