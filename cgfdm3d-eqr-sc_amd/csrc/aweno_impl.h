@@ -93,10 +93,12 @@ __device__ __forceinline__ void face_flux(const StageArgs<R>& a, long long p, lo
 // entry of F keeps the zero it was allocated with.
 template <class R>
 __global__ void __launch_bounds__(256) aw_faces(StageArgs<R> a) {
-  const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
+  // 64-column tiles start at i = 0 so that each wave's row segment starts on
+  // a cache line in the row-aligned layout; lanes with i < 2 have no face
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
   const int k = 2 + blockIdx.z;
-  if (i >= a.XE - 3 || j >= a.NY - 3) return;
+  if (i < 2 || i >= a.XE - 3 || j >= a.NY - 3) return;
   const long long p = (long long)i + (long long)a.NX * j + (long long)k * a.S;
   const bool ii = i >= H, ji = j >= H, ki = k >= H;
   if (ji && ki) face_flux<R>(a, p, 1, 0);
@@ -439,7 +441,7 @@ __global__ void aw_record(const R* __restrict__ W, const R* __restrict__ cjm, lo
 
 template <class R>
 void launch_faces(const StageArgs<R>& a, hipStream_t st) {
-  const dim3 grid((a.XE - 5 + 63) / 64, (a.NY - 5 + 3) / 4, a.NZ - 5), block(256);
+  const dim3 grid((a.XE - 3 + 63) / 64, (a.NY - 5 + 3) / 4, a.NZ - 5), block(256);
   aw_faces<R><<<grid, block, 0, st>>>(a);
 }
 

@@ -178,13 +178,17 @@ struct SolverBase {
   // allocated with spare elements and offset so that the first interior
   // column (i = 3) of every row sits on a 128-byte boundary (with a row pitch
   // that is a multiple of 32 elements). `galign` = false keeps offset 0.
+  // `gfirst` is the column whose rows start on a line: 3 for the CGFDM
+  // kernels (tiles start at the first interior cell), 0 for the ME-AWENO
+  // kernels (tiles cover the ghost columns too).
   bool galign = true;
+  int gfirst = 3;
   template <class X>
   int galloc(X** p, long long count) {
     X* q = nullptr;
     const long long line = 128 / (long long)sizeof(X);
     if (int rc = dalloc(&q, count + 2 * line)) return rc;
-    *p = q + (galign ? (line - 3) % line : 0);
+    *p = q + (galign ? (line - gfirst) % line : 0);
     return CGFDM_OK;
   }
 
@@ -1047,6 +1051,7 @@ struct AwenoSolver : SolverBase {
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
     if (const char* e = std::getenv("CGFDM_ROW_ALIGN")) galign = std::atoi(e) != 0;
+    gfirst = 0;
     tNX = d.nx + 2 * cg::H;
     NX = tNX;
     NY = d.ny + 2 * cg::H;
