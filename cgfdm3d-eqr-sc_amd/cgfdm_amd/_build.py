@@ -35,7 +35,7 @@ def _compile(src):
 
 
 HOST = os.path.join(ROOT, "host")
-HOST_SOURCES = ["json_lite.cpp", "params.cpp", "model_build.cpp", "source_build.cpp", "io_files.cpp"]
+HOST_SOURCES = ["json_lite.cpp", "params.cpp", "model_build.cpp", "source_build.cpp", "terrain.cpp", "io_files.cpp"]
 RUNNER = os.path.join(PKG, "cgfdm_run")
 
 

@@ -26,6 +26,7 @@
 #include "model_build.h"
 #include "params.h"
 #include "source_build.h"
+#include "terrain.h"
 
 using namespace cgh;
 
@@ -146,7 +147,16 @@ int main(int argc, char** argv) {
 
   Grid g = make_grid(p);
   std::vector<float> coord, terrain, medium;
-  if ((rc = build_coordinates(p, g, coord, terrain, msg))) return die(rc, msg);
+  {
+    SrtmResult srtm;
+    if ((rc = build_coordinates(p, g, coord, terrain, msg, &srtm))) return die(rc, msg);
+    if (p.useTerrain && p.SRTM90 == 1) {
+      // outputTotalTerrain, outputLonLat, outputTerrain (terrain.cpp:28-118, 825-833)
+      if ((rc = write_total_terrain(p, srtm.total, msg)) || (rc = write_surface_files(p, "lon", srtm.lon, msg)) ||
+          (rc = write_surface_files(p, "lat", srtm.lat, msg)) || (rc = write_surface_files(p, "terrain", terrain, msg)))
+        return die(rc, msg);
+    }
+  }
   if ((rc = build_medium(p, g, coord, terrain, medium, msg))) return die(rc, msg);
   if (!quiet) std::printf("grid %d x %d x %d, DH %g, DT %g, NT %d, nPML %d\n", p.NX, p.NY, p.NZ, p.DH, p.DT, NT, p.nPML);
 

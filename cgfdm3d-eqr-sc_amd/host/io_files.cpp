@@ -175,6 +175,32 @@ int write_source_files(const Params& p, const Grid& g, const std::vector<long lo
   return 0;
 }
 
+int write_surface_files(const Params& p, const std::string& name, const std::vector<float>& plane, std::string& msg) {
+  const int PZ = p.PZ > 0 ? p.PZ : 1, NXp = p.NX + 2 * HALO;
+  for (const Rank& r : ranks_of(p)) {
+    if (r.Z != PZ - 1) continue;
+    std::vector<float> v((size_t)r.sx.n * r.sy.n);
+    for (int j = 0; j < r.sy.n; ++j)
+      for (int i = 0; i < r.sx.n; ++i)
+        v[(size_t)j * r.sx.n + i] = plane[(size_t)(r.sx.front + i + HALO) + (size_t)NXp * (r.sy.front + j + HALO)];
+    std::string path = p.OUT + "/" + name + rank_suffix(r);
+    if (!write_file(path, v.data(), v.size() * sizeof(float))) {
+      msg = "cannot write " + path;
+      return 2;
+    }
+  }
+  return 0;
+}
+
+int write_total_terrain(const Params& p, const std::vector<float>& total, std::string& msg) {
+  std::string path = p.OUT + "/totalTerrain.bin";
+  if (!write_file(path, total.data(), total.size() * sizeof(float))) {
+    msg = "cannot write " + path;
+    return 2;
+  }
+  return 0;
+}
+
 int write_source_coords(const Params& p, const std::vector<float>& xyz, std::string& msg) {
   const size_t n = xyz.size() / 3;
   const char* tag[3] = {"X", "Y", "Z"};

@@ -60,6 +60,12 @@ int write_source_files(const Params& p, const Grid& g, const std::vector<long lo
 // source_coord_{X,Y,Z}.bin (verifyLocation, multiSource.cpp:641-672): float[npts] each.
 int write_source_coords(const Params& p, const std::vector<float>& xyz, std::string& msg);
 
+// "<OUT>/<name>_mpi_X_Y_Z.bin" of the surface ranks: float[ny][nx] of the rank's
+// interior columns from a padded plane [NY][NX] (outputTerrain / outputLonLat,
+// terrain.cpp:42-118), and "<OUT>/totalTerrain.bin" (outputTotalTerrain :28-40).
+int write_surface_files(const Params& p, const std::string& name, const std::vector<float>& plane, std::string& msg);
+int write_total_terrain(const Params& p, const std::vector<float>& total, std::string& msg);
+
 std::string rank_suffix(const Rank& r);  // "_mpi_X_Y_Z.bin"
 
 }  // namespace cgh

@@ -1,5 +1,7 @@
 #include "model_build.h"
 
+#include "terrain.h"
+
 #include <cmath>
 #include <cstdio>
 
@@ -23,11 +25,7 @@ Grid make_grid(const Params& p) {
 }
 
 int build_coordinates(const Params& p, const Grid& g, std::vector<float>& coord, std::vector<float>& terrain,
-                      std::string& msg) {
-  if (p.useTerrain) {
-    msg = "useTerrain: SRTM tiles and their reader are not part of this build (set useTerrain 0)";
-    return 2;
-  }
+                      std::string& msg, SrtmResult* srtm) {
   const long long N = g.cells();
   coord.assign((size_t)(3 * N), 0.f);
   terrain.assign((size_t)g.plane(), 0.f);
@@ -59,6 +57,16 @@ int build_coordinates(const Params& p, const Grid& g, std::vector<float>& coord,
           float DZ = (float)((double)(height + std::fabs(cal_depth)) / (double)(NZ - 1));
           coord[(size_t)(2 * N + g.idx(i, j, k))] = -std::fabs(cal_depth) + DZ * (float)(k - HALO);
         }
+  }
+  if (p.useTerrain) {
+    // preprocessTerrain (terrain.cpp:772-860)
+    if (p.SRTM90 == 1) {
+      SrtmResult local;
+      SrtmResult& r = srtm ? *srtm : local;
+      if (int rc = build_srtm_terrain(p, g, r, msg)) return rc;
+      terrain = r.terrain;
+    }
+    terrain_following_z(p, g, terrain, coord);
   }
   return 0;
 }

@@ -32,14 +32,18 @@ struct Grid {
 };
 Grid make_grid(const Params& p);
 
+struct SrtmResult;
+
 // Coordinates float[3][NZ][NY][NX] (x, y, z in metres) and the surface height
 // float[NY][NX] of the padded grid (coord.cpp:259-333): the flat grid
 // (construct_flat_coord, :100-135), optionally bent to the Gaussian hill
-// (construct_gauss_hill_surface :137-170 and construct_terrain_coord :205-257).
-// SRTM terrain (useTerrain) needs the tile files and their reader, which this
-// build does not have: returns 2 with a message.
+// (construct_gauss_hill_surface :137-170 and construct_terrain_coord :205-257),
+// and with useTerrain the terrain-following grid of preprocessTerrain
+// (terrain.cpp:772-860) over the SRTM surface when SRTM90 is 1 (`srtm`, when
+// given, receives the surface longitude/latitude and the tile map). Returns 0
+// or the reference's abort code with a message.
 int build_coordinates(const Params& p, const Grid& g, std::vector<float>& coord, std::vector<float>& terrain,
-                      std::string& msg);
+                      std::string& msg, SrtmResult* srtm = nullptr);
 
 // Medium float[3][NZ][NY][NX] (Vs, Vp, rho) (medium.cpp:251-317): homogeneous
 // Vs 3300, Vp 5716, rho 2700 (construct_homo_medium :84-105), replaced by the

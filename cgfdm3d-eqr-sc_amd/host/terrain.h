@@ -1,0 +1,44 @@
+// SRTM terrain for the terrain-following grid (reference terrain.cpp:206-860):
+// 5-degree tiles of 6000 x 6000 float elevations, calibration and smoothing
+// of the assembled map, interpolation onto the grid's surface through the
+// inverse azimuthal-equidistant projection, and the terrain-following
+// vertical coordinate.
+#ifndef CGFDM_HOST_TERRAIN_H
+#define CGFDM_HOST_TERRAIN_H
+
+#include <string>
+#include <vector>
+
+#include "model_build.h"
+#include "params.h"
+
+namespace cgh {
+
+constexpr int SRTM_POINTS = 6000;  // pointsPerBlockX/Y (terrain.cpp:20-21)
+constexpr int SRTM_DEGREES = 5;    // degreePerBlcokX/Y (terrain.cpp:17-18)
+
+// Inverse of aeqd_forward: the point at distance hypot(x, y) from (lon0, lat0)
+// along the azimuth atan2(x, y), from Vincenty's direct method (T. Vincenty,
+// Survey Review 23(176), 1975), iterated to 1e-13 rad. Degrees in and out.
+bool aeqd_inverse(double lon0, double lat0, double x, double y, double a, double f, double& lon, double& lat);
+
+struct SrtmResult {
+  std::vector<float> terrain;       // surface height [NY][NX] of the padded grid
+  std::vector<float> lon, lat;      // surface longitude / latitude [NY][NX] (projTrans, terrain.cpp:490-546)
+  std::vector<float> total;         // calibrated tile map [totalY][totalX] (totalTerrain.bin)
+  int totalX = 0, totalY = 0;
+};
+
+// readSRTM90 + resetTotalTerrain + callibrateTotalTerrain + projTrans +
+// terrainInterp (terrain.cpp:206-770) for the one-rank grid. Tiles are read
+// from "<TerrainDir>/srtm_<lat>N<lon>E.bin". Returns 0, or 1001 with a message
+// for a missing tile or a grid outside the tiles (the reference's abort code).
+int build_srtm_terrain(const Params& p, const Grid& g, SrtmResult& out, std::string& msg);
+
+// Terrain-following z of preprocessTerrain (terrain.cpp:838-858):
+// z = -|Depth| + (h + |Depth|) / (NZ - 1) * (k - 3), with h the surface height.
+void terrain_following_z(const Params& p, const Grid& g, const std::vector<float>& terrain, std::vector<float>& coord);
+
+}  // namespace cgh
+
+#endif

@@ -10,6 +10,7 @@
 #include "model_build.h"
 #include "params.h"
 #include "source_build.h"
+#include "terrain.h"
 
 using namespace cgh;
 
@@ -68,6 +69,10 @@ int h_model(const char* path, float* coord, float* terrain, float* medium) {
 
 int h_aeqd(double lon0, double lat0, double lon, double lat, double a, double f, double* xy) {
   return aeqd_forward(lon0, lat0, lon, lat, a, f, xy[0], xy[1]) ? 0 : 1;
+}
+
+int h_aeqd_inv(double lon0, double lat0, double x, double y, double a, double f, double* ll) {
+  return aeqd_inverse(lon0, lat0, x, y, a, f, ll[0], ll[1]) ? 0 : 1;
 }
 
 void h_double_couple(float s, float d, float r, float* m) { double_couple(s, d, r, m); }

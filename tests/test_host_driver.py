@@ -23,7 +23,7 @@ LIB = os.path.join(BUILD, "libhost_capi.so")
 GOLD = os.path.join(ROOT, "tests", "golden", "reference_example")
 RUNNER = os.path.join(ROOT, "cgfdm3d-eqr-sc_amd", "cgfdm_amd", "cgfdm_run")
 H = 3
-SRCS = ["json_lite.cpp", "params.cpp", "model_build.cpp", "source_build.cpp", "io_files.cpp"]
+SRCS = ["json_lite.cpp", "params.cpp", "model_build.cpp", "source_build.cpp", "terrain.cpp", "io_files.cpp"]
 
 
 _LIB = []
@@ -144,8 +144,9 @@ def test_flat_grid_and_homogeneous_medium(lib, tmp_path):
     assert np.allclose(np.diff(c[2], axis=0), 100)
     assert np.all(t == 0)
     assert np.all(m[0] == 3300) and np.all(m[1] == 5716) and np.all(m[2] == 2700)
-    # SRTM terrain is not available: refused with a message
-    assert model(lib, write_params(tmp_path, useTerrain=1), 20, 18, 16)[0] == 2
+    # SRTM terrain without its tiles: the reference's abort code 1001
+    assert model(lib, write_params(tmp_path, useTerrain=1, SRTM90=1, TerrainDir=str(tmp_path / "none"),
+                                   lonStart=100, latStart=35, blockX=1, blockY=1), 20, 18, 16)[0] == 1001
 
 
 def test_gauss_hill_grid(lib, tmp_path):
@@ -524,5 +525,75 @@ def test_runner_error_codes(tmp_path):
     assert r.returncode == 180
     r = run(**{"useSingleSource(ricker)": 0})
     assert r.returncode == 180 and "source" in r.stderr
-    r = run(useTerrain=1)
-    assert r.returncode == 2 and "SRTM" in r.stderr
+    r = run(useTerrain=1, SRTM90=1, TerrainDir="none", lonStart=100, latStart=35, blockX=1, blockY=1)
+    assert r.returncode == 1001 % 256 and "no such file" in r.stderr
+
+
+# ---------------------------------------------------------------------------
+# SRTM terrain
+# ---------------------------------------------------------------------------
+def aeqd_inv(lib, lon0, lat0, x, y, a=6378137.0, f=1 / 298.257223563):
+    ll = (ctypes.c_double * 2)()
+    assert lib.h_aeqd_inv(lon0, lat0, x, y, a, f, ll) == 0
+    return ll[0], ll[1]
+
+
+def test_aeqd_inverse_round_trip_and_published_direct(lib):
+    lib.h_aeqd_inv.argtypes = [ctypes.c_double] * 6 + [ctypes.POINTER(ctypes.c_double)]
+    for x, y in [(0.0, 1000.0), (25000.0, -13000.0), (-80000.0, 40000.0), (3.0, 0.5)]:
+        lon, lat = aeqd_inv(lib, 101.25, 37.79, x, y)
+        x2, y2 = aeqd(lib, 101.25, 37.79, lon, lat)
+        assert x2 == pytest.approx(x, abs=1e-6) and y2 == pytest.approx(y, abs=1e-6)
+    # Flinders Peak, azimuth 306 52' 05.37", 54972.271 m -> Buninyong (GRS80)
+    def dms(d, m, s):
+        return math.copysign(abs(d) + m / 60 + s / 3600, d)
+    az = math.radians(dms(306, 52, 5.37))
+    s = 54972.271
+    lon, lat = aeqd_inv(lib, dms(144, 25, 29.52440), dms(-37, 57, 3.72030), s * math.sin(az), s * math.cos(az),
+                        f=1 / 298.257222101)
+    assert lat == pytest.approx(dms(-37, 39, 10.15610), abs=2e-8)
+    assert lon == pytest.approx(dms(143, 55, 35.38390), abs=2e-8)
+
+
+def test_srtm_terrain_from_a_ramp_tile(lib, tmp_path):
+    lib.h_aeqd_inv.argtypes = [ctypes.c_double] * 6 + [ctypes.POINTER(ctypes.c_double)]
+    P = 6000
+    J = np.arange(P, dtype=np.float64)
+    ramp = (100.0 + 2000.0 * J / (P - 1)).astype(np.float32)     # rises with the tile row (latitude)
+    tile = np.repeat(ramp[:, None], P, axis=1)                  # [row = lat][col = lon]
+    tile[P // 2 + 40, P // 2 + 30] += 5000.0                     # one spike under the grid
+    tdir = tmp_path / "tiles"
+    os.makedirs(tdir)
+    tile.tofile(tdir / "srtm_35N100E.bin")
+    nx, ny, nz = 30, 26, 12
+    path = write_params(tmp_path, NX=nx, NY=ny, NZ=nz, DH=90.0, centerX=15, centerY=13, useTerrain=1, SRTM90=1,
+                        TerrainDir=str(tdir), lonStart=100, latStart=35, blockX=1, blockY=1,
+                        centerLongitude=102.5, centerLatitude=37.5, **{"Depth(km)": 2.0})
+    rc, c, t, m = model(lib, path, nx, ny, nz)
+    assert rc == 0, lib.h_last_msg()
+    # closed form: the reference maps latitude to the fractional tile row with a spacing of
+    # 5/6000 degree, the tile holds a linear ramp in the row, bilinear interpolation and
+    # the symmetric Gaussian smoothings (weights summing to F each) keep a linear field
+    g1 = np.exp(-((np.arange(11) - 5) / 2.5) ** 2) / (2.5 * 1.772453850905516)
+    F = g1.sum() ** 2 / 0.996768
+    refdeg = math.pi / float(np.float32(3.141592657))
+    worst = 0.0
+    for j in range(0, ny + 6, 5):
+        for i in range(0, nx + 6, 7):
+            x = float(np.float32((i - 3) * 90.0) - np.float32(15 * 90.0))
+            y = float(np.float32((j - 3) * 90.0) - np.float32(13 * 90.0))
+            lon, lat = aeqd_inv(lib, 102.5, 37.5, x, y)
+            row = (lat * refdeg - 35.0) / (5.0 / P)
+            expect = F * F * (100.0 + 2000.0 * row / (P - 1))
+            worst = max(worst, abs(t[j, i] - expect))
+    assert worst < 0.05, worst
+    # the spike is gone (despiked where the central gradient exceeds 100 m, then smoothed)
+    assert t.max() < 100.0 + 2000.0 * (37.6 - 35.0) / 5.0 * 1.01
+    # terrain-following grid from -Depth to the surface
+    assert np.allclose(c[2, 3], -2000.0, atol=1e-3)
+    assert np.allclose(c[2, 3 + nz - 1], t, atol=1e-2)
+    # a grid reaching beyond the tile is refused with 1001
+    path = write_params(tmp_path, NX=nx, NY=ny, NZ=nz, DH=90.0, centerX=15, centerY=13, useTerrain=1, SRTM90=1,
+                        TerrainDir=str(tdir), lonStart=100, latStart=35, blockX=1, blockY=1,
+                        centerLongitude=100.001, centerLatitude=37.5, **{"Depth(km)": 2.0})
+    assert model(lib, path, nx, ny, nz)[0] == 1001
