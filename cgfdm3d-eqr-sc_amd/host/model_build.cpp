@@ -112,9 +112,12 @@ int build_medium(const Params& p, const Grid& g, const std::vector<float>& coord
     medium[(size_t)(2 * N + c)] = 2700.0f;
   }
   if (!p.useMedium) return 0;
-  if (p.Crust_1Model || p.ShenModel) {
-    msg = "ShenModel / Crust_1Model: the data sets and their readers are not part of this build";
+  if (p.Crust_1Model) {
+    msg = "Crust_1Model: the CRUST1.0 data set and its reader are not part of this build";
     return 2;
+  }
+  if (p.ShenModel) {
+    if (int rc = build_shen_medium(p, g, coord, medium, msg)) return rc;
   }
   if (!p.LayeredModel) return 0;
   std::vector<float> H, VS, VP, RHO;

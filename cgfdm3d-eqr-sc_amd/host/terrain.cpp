@@ -249,6 +249,115 @@ int build_srtm_terrain(const Params& p, const Grid& g, SrtmResult& out, std::str
   return 0;
 }
 
+int surface_lonlat(const Params& p, const Grid& g, float lon0, float lon1, float lat0, float lat1,
+                   std::vector<double>& lon, std::vector<double>& lat, std::string& msg) {
+  const double c0 = fmt6(p.centerLongitude), c1 = fmt6(p.centerLatitude);
+  const float DH = (float)p.DH;
+  const int oX = p.centerX + HALO, oY = p.centerY + HALO;
+  const double xs[2] = {-(oX + 2) * (double)DH, ((g.NX - oX) + 2) * (double)DH};
+  const double ys[2] = {-(oY + 2) * (double)DH, ((g.NY - oY) + 2) * (double)DH};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      double lo, la;
+      aeqd_inverse(c0, c1, xs[a], ys[b], WGS84_A, WGS84_F, lo, la);
+      const float L0 = (float)(lo * TO_REF_DEG), L1 = (float)(la * TO_REF_DEG);
+      if (L0 > lon1 || L0 < lon0 || L1 > lat1 || L1 < lat0) {
+        msg = "The longitude or latitude range of the calculation extends beyond the terrain or medium data";
+        return 1001;
+      }
+    }
+  lon.assign((size_t)g.plane(), 0.0);
+  lat.assign((size_t)g.plane(), 0.0);
+  for (int j = 0; j < g.NY; ++j)
+    for (int i = 0; i < g.NX; ++i) {
+      const double x = (double)((float)(i - HALO) * DH - (float)p.centerX * DH);
+      const double y = (double)((float)(j - HALO) * DH - (float)p.centerY * DH);
+      double lo, la;
+      aeqd_inverse(c0, c1, x, y, WGS84_A, WGS84_F, lo, la);
+      lon[(size_t)g.idx(i, j, 0)] = lo * TO_REF_DEG;
+      lat[(size_t)g.idx(i, j, 0)] = la * TO_REF_DEG;
+    }
+  return 0;
+}
+
+int build_shen_medium(const Params& p, const Grid& g, const std::vector<float>& coord, std::vector<float>& medium,
+                      std::string& msg) {
+  const float LonStart = p.MLonStart, LatStart = p.MLatStart, LonEnd = p.MLonEnd, LatEnd = p.MLatEnd;
+  const float LonStep = p.MLonStep, LatStep = p.MLatStep;
+  if (!(LonStep > 0) || !(LatStep > 0) || !(p.MVeticalStep > 0)) {
+    msg = "MLonStep, MLatStep and MVeticalStep must be positive";
+    return 1001;
+  }
+  const int nLon = int((LonEnd - LonStart) / LonStep + 0.5 + 1), nLat = int((LatEnd - LatStart) / LatStep + 0.5 + 1);
+  const float depthAll = p.Depth * 1e3f + 3e4f;  // Deeper (dealMedium.cpp:201)
+  const int NLayer = int(depthAll / p.MVeticalStep);
+  std::vector<float> VM((size_t)nLon * nLat * NLayer);
+  for (int j = 0; j < nLat; ++j)
+    for (int i = 0; i < nLon; ++i) {
+      char ln[64], la[64], name[1024];
+      std::snprintf(ln, sizeof ln, "%.1f", LonStart + i * LonStep);
+      std::snprintf(la, sizeof la, "%.1f", LatStart + j * LatStep);
+      std::snprintf(name, sizeof name, "%s/%s_%s.mod", p.MediumDir.c_str(), ln, la);
+      FILE* f = std::fopen(name, "r");
+      if (!f) {
+        msg = std::string("Can't open file: ") + name;
+        return 1001;
+      }
+      for (int K = 0; K < NLayer; ++K) {
+        double dep, vs, err;
+        if (std::fscanf(f, "%lf %lf %lf", &dep, &vs, &err) != 3) {
+          std::fclose(f);
+          msg = std::string("model file ") + name + " has fewer than " + std::to_string(NLayer) + " layers";
+          return 1001;
+        }
+        VM[(size_t)K + (size_t)(i + j * nLon) * NLayer] = (float)(vs * 1e3);
+      }
+      std::fclose(f);
+    }
+  std::vector<double> lon, lat;
+  if (int rc = surface_lonlat(p, g, LonStart, LonEnd, LatStart, LatEnd, lon, lat, msg)) return rc;
+  // structureInterp: bilinear per layer at the surface points
+  const size_t P2 = (size_t)g.plane();
+  std::vector<float> Vs(P2 * NLayer);
+  for (int K = 0; K < NLayer; ++K)
+    for (size_t q = 0; q < P2; ++q) {
+      const double xq = lon[q], yq = lat[q];
+      int I = int((xq - LonStart) / LonStep), J = int((yq - LatStart) / LatStep);
+      I = std::max(0, std::min(I, nLon - 2));
+      J = std::max(0, std::min(J, nLat - 2));
+      const double x1 = I * LonStep + LonStart, y1 = J * LatStep + LatStart, x2 = x1 + LonStep, y2 = y1 + LatStep;
+      auto V = [&](int a, int b) { return (double)VM[(size_t)K + (size_t)(a + b * nLon) * NLayer]; };
+      const double Lx0 = (xq - x2) / (x1 - x2), Lx1 = (xq - x1) / (x2 - x1);
+      const double Ly0 = (yq - y2) / (y1 - y2), Ly1 = (yq - y1) / (y2 - y1);
+      Vs[(size_t)K * P2 + q] =
+          (float)(Lx0 * Ly0 * V(I, J) + Lx0 * Ly1 * V(I, J + 1) + Lx1 * Ly0 * V(I + 1, J) + Lx1 * Ly1 * V(I + 1, J + 1));
+    }
+  if (NLayer > 1)  // DealWithFirstLayer
+    for (size_t q = 0; q < P2; ++q) Vs[q] = Vs[P2 + q];
+  const long long N = g.cells();
+  const double Depth = (double)p.Depth * 1e3;
+  for (int k = 0; k < g.NZ; ++k)
+    for (int j = 0; j < g.NY; ++j)
+      for (int i = 0; i < g.NX; ++i) {
+        const long long c = g.idx(i, j, k);
+        // StructureTerrain: depth below the column's surface DZ (NZ - 1) - |Depth|
+        const double DZ = (double)(float)(coord[(size_t)(2 * N + g.idx(i, j, 4))] - coord[(size_t)(2 * N + g.idx(i, j, 3))]);
+        const double ter = DZ * double(p.NZ - 1) - std::fabs(Depth);
+        int K = (int)((float)int(-(coord[(size_t)(2 * N + c)] - ter)) / p.MVeticalStep);
+        if (K <= 0) K = 0;
+        if (K >= NLayer) K = NLayer - 1;
+        const float VS = Vs[(size_t)K * P2 + (size_t)g.idx(i, j, 0)] * 1e-3f;
+        const float VP = (float)(0.9409 + 2.0947 * VS - 0.8206 * std::pow(VS, 2) + 0.2683 * std::pow(VS, 3) -
+                                 0.0251 * std::pow(VS, 4));
+        const float RHO = (float)(1.6612 * VP - 0.4721 * std::pow(VP, 2) + 0.0671 * std::pow(VP, 3) -
+                                  0.0043 * std::pow(VP, 4) + 0.000106 * std::pow(VP, 5));
+        medium[(size_t)c] = VS * 1e3f;
+        medium[(size_t)(N + c)] = VP * 1e3f;
+        medium[(size_t)(2 * N + c)] = RHO * 1e3f;
+      }
+  return 0;
+}
+
 void terrain_following_z(const Params& p, const Grid& g, const std::vector<float>& terrain, std::vector<float>& coord) {
   const long long N = g.cells();
   const double Depth = (double)p.Depth * 1e3;

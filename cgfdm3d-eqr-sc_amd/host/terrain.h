@@ -35,6 +35,23 @@ struct SrtmResult {
 // for a missing tile or a grid outside the tiles (the reference's abort code).
 int build_srtm_terrain(const Params& p, const Grid& g, SrtmResult& out, std::string& msg);
 
+// Surface longitude/latitude of the grid points (projTrans / cart2LonLat,
+// terrain.cpp:450-546) in the reference's degree convention, and its check that
+// the grid's corners, widened by 2 cells, lie inside [lon0, lon1] x [lat0, lat1]
+// (1001 otherwise).
+int surface_lonlat(const Params& p, const Grid& g, float lon0, float lon1, float lat0, float lat1,
+                   std::vector<double>& lon, std::vector<double>& lat, std::string& msg);
+
+// Shen model medium (readWeisenShenModel / structureInterp, dealMedium.cpp:49-360):
+// "<MediumDir>/<lon>_<lat>.mod" text profiles of (depth, Vs, error) every
+// MVeticalStep metres on the MLonStart..MLonEnd x MLatStart..MLatEnd node grid,
+// interpolated to the surface points layer by layer, the first layer replaced by
+// the second (DealWithFirstLayer), looked up by the depth below the local surface
+// (StructureTerrain) and turned into Vp and rho by the empirical relations of the
+// reference. Returns 0, or 1001 with a message (missing file, grid outside).
+int build_shen_medium(const Params& p, const Grid& g, const std::vector<float>& coord, std::vector<float>& medium,
+                      std::string& msg);
+
 // Terrain-following z of preprocessTerrain (terrain.cpp:838-858):
 // z = -|Depth| + (h + |Depth|) / (NZ - 1) * (k - 3), with h the surface height.
 void terrain_following_z(const Params& p, const Grid& g, const std::vector<float>& terrain, std::vector<float>& coord);

@@ -597,3 +597,47 @@ def test_srtm_terrain_from_a_ramp_tile(lib, tmp_path):
                         TerrainDir=str(tdir), lonStart=100, latStart=35, blockX=1, blockY=1,
                         centerLongitude=100.001, centerLatitude=37.5, **{"Depth(km)": 2.0})
     assert model(lib, path, nx, ny, nz)[0] == 1001
+
+
+def test_shen_model_medium(lib, tmp_path):
+    lib.h_aeqd_inv.argtypes = [ctypes.c_double] * 6 + [ctypes.POINTER(ctypes.c_double)]
+    mdir = tmp_path / "shen"
+    os.makedirs(mdir)
+    nlayer = int((1.6e3 + 3e4) / 100.0)
+    for a in range(3):
+        for b in range(3):
+            lon, lat = 101.0 + 0.5 * a, 37.0 + 0.5 * b
+            K = np.arange(nlayer + 5)
+            vs = 2.0 + 0.01 * K + 0.5 * (lon - 101.0)          # km/s, deeper is faster, eastward faster
+            rows = np.stack([K * 0.1, vs, np.full_like(vs, 0.01)], axis=1)
+            np.savetxt(mdir / f"{lon:.1f}_{lat:.1f}.mod", rows, fmt="%.6f")
+    nx, ny, nz = 20, 18, 16
+    path = write_params(tmp_path, NX=nx, NY=ny, NZ=nz, DH=100.0, centerX=10, centerY=9, useMedium=1, ShenModel=1,
+                        MLonStart=101.0, MLatStart=37.0, MLonEnd=102.0, MLatEnd=38.0, MLonStep=0.5, MLatStep=0.5,
+                        MVeticalStep=100.0, MediumDir=str(mdir), centerLongitude=101.5, centerLatitude=37.5)
+    rc, c, t, m = model(lib, path, nx, ny, nz)
+    assert rc == 0, lib.h_last_msg()
+    vs, vp, rho = m[0].astype(np.float64), m[1].astype(np.float64), m[2].astype(np.float64)
+    # one cell deeper is one 100 m layer deeper: +10 m/s (below the clamped top layers)
+    d = vs[3:12] - vs[4:13]
+    assert np.allclose(d, 10.0, atol=0.05)
+    # laterally: 500 m/s per degree of longitude, through the surface longitude of the column
+    refdeg = math.pi / float(np.float32(3.141592657))
+    for (i, j) in [(3, 3), (12, 7), (20, 15)]:
+        x = float(np.float32((i - 3) * 100.0) - np.float32(10 * 100.0))
+        y = float(np.float32((j - 3) * 100.0) - np.float32(9 * 100.0))
+        lon, lat = aeqd_inv(lib, 101.5, 37.5, x, y)
+        col = vs[5, j, i] - vs[5, 3 + 9, 3 + 10]
+        lon0, _ = aeqd_inv(lib, 101.5, 37.5, 0.0, 0.0)
+        assert col == pytest.approx(500.0 * (lon - lon0) * refdeg, abs=0.05)
+    # Vp and rho from Vs by Brocher's (2005) regressions (Vp(Vs) and Nafe-Drake rho(Vp), km/s, g/cm^3)
+    VS = vs / 1e3
+    VP = 0.9409 + 2.0947 * VS - 0.8206 * VS ** 2 + 0.2683 * VS ** 3 - 0.0251 * VS ** 4
+    RHO = 1.6612 * VP - 0.4721 * VP ** 2 + 0.0671 * VP ** 3 - 0.0043 * VP ** 4 + 0.000106 * VP ** 5
+    np.testing.assert_allclose(vp, VP * 1e3, rtol=2e-6)
+    np.testing.assert_allclose(rho, RHO * 1e3, rtol=2e-6)
+    # a model that does not cover the grid is refused with 1001
+    path = write_params(tmp_path, NX=nx, NY=ny, NZ=nz, DH=100.0, centerX=10, centerY=9, useMedium=1, ShenModel=1,
+                        MLonStart=101.0, MLatStart=37.0, MLonEnd=102.0, MLatEnd=38.0, MLonStep=0.5, MLatStep=0.5,
+                        MVeticalStep=100.0, MediumDir=str(mdir), centerLongitude=101.001, centerLatitude=37.5)
+    assert model(lib, path, nx, ny, nz)[0] == 1001
