@@ -113,8 +113,7 @@ int build_medium(const Params& p, const Grid& g, const std::vector<float>& coord
   }
   if (!p.useMedium) return 0;
   if (p.Crust_1Model) {
-    msg = "Crust_1Model: the CRUST1.0 data set and its reader are not part of this build";
-    return 2;
+    if (int rc = build_crust1_medium(p, g, coord, medium, msg)) return rc;
   }
   if (p.ShenModel) {
     if (int rc = build_shen_medium(p, g, coord, medium, msg)) return rc;

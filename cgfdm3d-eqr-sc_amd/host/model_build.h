@@ -46,11 +46,12 @@ int build_coordinates(const Params& p, const Grid& g, std::vector<float>& coord,
                       std::string& msg, SrtmResult* srtm = nullptr);
 
 // Medium float[3][NZ][NY][NX] (Vs, Vp, rho) (medium.cpp:251-317): homogeneous
-// Vs 3300, Vp 5716, rho 2700 (construct_homo_medium :84-105), replaced by the
-// layered model file when useMedium and LayeredModel are set
-// (generate_1D_medium :107-213, layers measured from the local surface height
-// as the reference build does with LayeredStructureTerrain). The Shen and
-// CRUST1.0 models read data sets this build does not have: returns 2.
+// Vs 3300, Vp 5716, rho 2700 (construct_homo_medium :84-105), then with
+// useMedium the CRUST1.0 model, the Shen model and the layered model file in
+// the reference's order, each overwriting the previous (readCrustal_1,
+// readWeisenShenModel, generate_1D_medium :107-213 with layers measured from the
+// local surface height as the reference build does with LayeredStructureTerrain).
+// Returns 0 or the reference's abort code with a message.
 int build_medium(const Params& p, const Grid& g, const std::vector<float>& coord, const std::vector<float>& terrain,
                  std::vector<float>& medium, std::string& msg);
 

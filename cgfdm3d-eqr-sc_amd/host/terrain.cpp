@@ -358,6 +358,89 @@ int build_shen_medium(const Params& p, const Grid& g, const std::vector<float>& 
   return 0;
 }
 
+int build_crust1_medium(const Params& p, const Grid& g, const std::vector<float>& coord, std::vector<float>& medium,
+                        std::string& msg) {
+  const int nLon = 360, nLat = 180, nL = 9, nLL = nLon * nLat;
+  const char* names[4] = {"crust1.vs", "crust1.vp", "crust1.rho", "crust1.bnds"};
+  std::vector<float> F[4];
+  for (int q = 0; q < 4; ++q) {
+    const std::string path = p.crustDir + "/" + names[q];
+    FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) {
+      msg = "Can't open file: " + path;
+      return 1001;
+    }
+    F[q].assign((size_t)nLL * nL, 0.f);
+    for (int i = 0; i < nLL; ++i)
+      for (int n = 0; n < nL; ++n)
+        if (std::fscanf(f, "%f", &F[q][(size_t)i + (size_t)n * nLL]) != 1) {
+          std::fclose(f);
+          msg = path + " is truncated";
+          return 1001;
+        }
+    std::fclose(f);
+  }
+  std::vector<double> lon, lat;
+  if (int rc = surface_lonlat(p, g, -180.f, 180.f, -90.f, 90.f, lon, lat, msg)) return rc;
+  // crustInterpFunc: bilinear between the cell centres around each surface point
+  const float LonStart = -179.5f, LatStart = 89.5f, LonStep = p.CrustLonStep, LatStep = p.CrustLatStep;
+  if (!(LonStep > 0) || !(LatStep > 0)) {
+    msg = "CrustLonStep and CrustLatStep must be positive";
+    return 1001;
+  }
+  const size_t P2 = (size_t)g.plane();
+  std::vector<float> I4[4];
+  for (int q = 0; q < 4; ++q) I4[q].assign(P2 * nL, 0.f);
+  for (int K = 0; K < nL; ++K)
+    for (size_t s2 = 0; s2 < P2; ++s2) {
+      const double xq = lon[s2], yq = lat[s2];
+      int I = int((xq - LonStart) / LonStep), J = int((LatStart - yq) / LatStep) + 1;
+      I = std::max(0, std::min(I, nLon - 2));
+      J = std::max(1, std::min(J, nLat - 1));
+      const double x1 = I * LonStep + LonStart, y1 = LatStart - J * LatStep, x2 = x1 + LonStep, y2 = y1 + LatStep;
+      const double Lx0 = (xq - x2) / (x1 - x2), Lx1 = (xq - x1) / (x2 - x1);
+      const double Ly0 = (yq - y2) / (y1 - y2), Ly1 = (yq - y1) / (y2 - y1);
+      for (int q = 0; q < 4; ++q) {
+        auto V = [&](int a, int b) { return (double)F[q][(size_t)(a + b * nLon) + (size_t)K * nLL]; };
+        I4[q][(size_t)K * P2 + s2] =
+            (float)(Lx0 * Ly0 * V(I, J) + Lx0 * Ly1 * V(I, J - 1) + Lx1 * Ly0 * V(I + 1, J) + Lx1 * Ly1 * V(I + 1, J - 1));
+      }
+    }
+  // solveStructure: boundaries in metres; the properties carry over from the
+  // previous cell where no rule applies, as in the reference
+  for (float& b : I4[3]) b *= 1e3f;
+  const long long N = g.cells();
+  float vs = 0, vp = 0, rho = 0;
+  auto bnd = [&](int l, size_t s2) { return I4[3][(size_t)l * P2 + s2]; };
+  auto take = [&](int l, size_t s2) {
+    vs = I4[0][(size_t)l * P2 + s2];
+    vp = I4[1][(size_t)l * P2 + s2];
+    rho = I4[2][(size_t)l * P2 + s2];
+  };
+  // first layer below `m` whose top differs from layer m's by 1 km or more, minus one
+  auto skip_thin = [&](int m, size_t s2) {
+    int l = m;
+    while (l < nL && std::fabs(bnd(m, s2) - bnd(l, s2)) < 1000.0f) ++l;
+    return l - 1;
+  };
+  for (int k = 0; k < g.NZ; ++k)
+    for (int j = 0; j < g.NY; ++j)
+      for (int i = 0; i < g.NX; ++i) {
+        const size_t s2 = (size_t)g.idx(i, j, 0);
+        const long long c = g.idx(i, j, k);
+        const float z = coord[(size_t)(2 * N + c)];
+        if (z > bnd(0, s2)) take(skip_thin(0, s2), s2);
+        if (z <= bnd(nL - 1, s2)) take(nL - 1, s2);
+        if (z <= bnd(0, s2) && z > bnd(nL - 1, s2))
+          for (int m = 0; m < nL - 1; ++m)
+            if (z <= bnd(m, s2) && z > bnd(m + 1, s2)) take(skip_thin(m, s2), s2);
+        medium[(size_t)c] = vs * 1e3f;
+        medium[(size_t)(N + c)] = vp * 1e3f;
+        medium[(size_t)(2 * N + c)] = rho * 1e3f;
+      }
+  return 0;
+}
+
 void terrain_following_z(const Params& p, const Grid& g, const std::vector<float>& terrain, std::vector<float>& coord) {
   const long long N = g.cells();
   const double Depth = (double)p.Depth * 1e3;

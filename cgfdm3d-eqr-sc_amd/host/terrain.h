@@ -52,6 +52,15 @@ int surface_lonlat(const Params& p, const Grid& g, float lon0, float lon1, float
 int build_shen_medium(const Params& p, const Grid& g, const std::vector<float>& coord, std::vector<float>& medium,
                       std::string& msg);
 
+// CRUST1.0 medium (readCrustal_1 / crustInterpFunc / solveStructure,
+// crustMedium.cpp): "<CrustDir>/crust1.{vs,vp,rho,bnds}" text files of 9 layer
+// values per 1-degree cell (360 x 180, longitude fastest from -179.5, latitude
+// rows from 89.5 southwards), bilinear at the surface points, then per grid
+// cell the properties of the layer whose top lies above it, skipping layers
+// thinner than 1 km as the reference does. Returns 0 or 1001 with a message.
+int build_crust1_medium(const Params& p, const Grid& g, const std::vector<float>& coord, std::vector<float>& medium,
+                        std::string& msg);
+
 // Terrain-following z of preprocessTerrain (terrain.cpp:838-858):
 // z = -|Depth| + (h + |Depth|) / (NZ - 1) * (k - 3), with h the surface height.
 void terrain_following_z(const Params& p, const Grid& g, const std::vector<float>& terrain, std::vector<float>& coord);

@@ -641,3 +641,45 @@ def test_shen_model_medium(lib, tmp_path):
                         MLonStart=101.0, MLatStart=37.0, MLonEnd=102.0, MLatEnd=38.0, MLonStep=0.5, MLatStep=0.5,
                         MVeticalStep=100.0, MediumDir=str(mdir), centerLongitude=101.001, centerLatitude=37.5)
     assert model(lib, path, nx, ny, nz)[0] == 1001
+
+
+def test_crust1_medium(lib, tmp_path):
+    lib.h_aeqd_inv.argtypes = [ctypes.c_double] * 6 + [ctypes.POINTER(ctypes.c_double)]
+    cdir = tmp_path / "crust"
+    os.makedirs(cdir)
+    nlon, nlat, nl = 360, 180, 9
+    tops = np.array([0.0, -0.5, -3.0, -3.0, -4.5, -6.0, -7.5, -9.0, -12.0])   # km; layer 0 is 0.5 km thin, 2 is empty
+    I = np.arange(nlon)[None, :].repeat(nlat, 0).ravel()                    # longitude cell of each record
+    base = 1.0 + 0.5 * np.arange(nl)
+    vs = base[None, :] + 0.01 * I[:, None]                                   # eastward gradient, per layer
+    vp = 1.8 * vs
+    rho = 2.0 + 0.1 * np.arange(nl)[None, :] + 0 * I[:, None]
+    bnds = tops[None, :] + 0 * I[:, None]
+    for name, arr in (("vs", vs), ("vp", vp), ("rho", rho), ("bnds", bnds)):
+        np.savetxt(cdir / f"crust1.{name}", arr, fmt="%.5f")
+    nx, ny, nz = 20, 18, 40
+    path = write_params(tmp_path, NX=nx, NY=ny, NZ=nz, DH=300.0, centerX=10, centerY=9, useMedium=1, Crust_1Model=1,
+                        CrustDir=str(cdir), CrustLonStep=1.0, CrustLatStep=1.0, centerLongitude=101.5,
+                        centerLatitude=37.5)
+    rc, c, t, m = model(lib, path, nx, ny, nz)
+    assert rc == 0, lib.h_last_msg()
+    refdeg = math.pi / float(np.float32(3.141592657))
+    H3 = 3
+    for (i, j) in [(3, 3), (15, 11), (22, 20)]:
+        x = float(np.float32((i - 3) * 300.0) - np.float32(10 * 300.0))
+        y = float(np.float32((j - 3) * 300.0) - np.float32(9 * 300.0))
+        lon, lat = aeqd_inv(lib, 101.5, 37.5, x, y)
+        dI = (lon * refdeg - (-179.5)) * 0.01                                # interpolated eastward term
+        for k in range(H3, H3 + nz):
+            z = c[2, k, j, i] / 1e3
+            # the layer whose top is the lowest one above the cell; layer 0 is thinner than 1 km,
+            # so its cells (and any above the surface) take layer 1's values; the empty layer 2
+            # (same top as layer 3) is never chosen
+            if z > -0.5:
+                L = 1
+            elif z <= -12.0:
+                L = 8
+            else:
+                L = max(q for q in range(nl - 1) if tops[q] >= z > tops[q + 1])
+            assert m[0, k, j, i] == pytest.approx((base[L] + dI) * 1e3, abs=0.5), (i, j, k, z, L)
+            assert m[2, k, j, i] == pytest.approx((2.0 + 0.1 * L) * 1e3, abs=0.05)
