@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--scheme", choices=["cgfdm", "aweno"], default="cgfdm")
     ap.add_argument("--adjoint", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the short forward+adjoint measurement of the default run")
     ap.add_argument("--checkpoint", type=int, default=20)
     ap.add_argument("--adjoint-fd", type=int, default=0, metavar="NDIR",
                     help="with --adjoint: check the gradient against central differences in NDIR random directions")
@@ -221,11 +223,25 @@ def main():
                      "alg_bytes_per_cell": stage_bytes(args.coef), "cells_per_launch": cells_launch},
         "cpu_baseline": cpu,
     }
+    if not args.no_adjoint:
+        # the metric's second part: forward + adjoint against the plain forward, on
+        # the adjoint-gradient workload (a short run; `bench.py --adjoint` for more steps)
+        ma, t_fwd, t_rec, t_adj, ok = measure_adjoint(20, 2, 10)
+        out["fwd_adjoint"] = {
+            "cost_ratio_fwd_adj_vs_fwd": (t_rec + t_adj) / t_fwd,
+            "fwd_adj_gcells": ma.cells * 20 / (t_rec + t_adj) / 1e9,
+            "forward_gcells": ma.cells * 20 / t_fwd / 1e9,
+            "gradients_finite": ok,
+            "workload": f"CGFDM forward+adjoint {ma.nx}x{ma.ny}x{ma.nz} h=100m, 4 hills, 4 layers + 5% random, "
+                        f"PML 20, 16x8 finite fault, {len(ma.stations)} stations, 20 steps, checkpoint every 10",
+        }
     print(json.dumps(out), flush=True)
 
 
-def bench_adjoint(args):
-    """forward_record + adjoint vs plain forward steps, FP32, same workload."""
+def measure_adjoint(steps, warmup, checkpoint):
+    """Plain forward steps against forward_record + adjoint of the same steps on the
+    adjoint-gradient workload (256 x 256 x 128, terrain, layered + random medium,
+    16 x 8 finite fault, 64 stations), FP32. Returns the model and the three times."""
     import numpy as np
     import torch
     from cgfdm_amd import model as M
@@ -233,30 +249,38 @@ def bench_adjoint(args):
 
     nx, ny, nz = 256, 256, 128
     hmin_gpu = lambda c, md: S.grid_ranges(nx, ny, nz, c, md)["hmin"]
-    m = M.config5(nt=args.steps + args.warmup + 1, hmin_fn=hmin_gpu)
+    m = M.config5(nt=steps + warmup + 1, hmin_fn=hmin_gpu)
     log(f"adjoint model {m.nx}x{m.ny}x{m.nz}, {len(m.src_cells)} fault points, {len(m.stations)} stations")
     g = S.Solver(m, precision=4)
-    g.step(args.warmup)
+    g.step(warmup)
     g.synchronize()
     t0 = time.perf_counter()
-    g.step(args.steps)
+    g.step(steps)
     g.synchronize()
     t_fwd = time.perf_counter() - t0
     g.close()
     torch.cuda.empty_cache()
     a = S.Solver(m, precision=4)
-    a.step(args.warmup)
+    a.step(warmup)
     a.synchronize()
     t0 = time.perf_counter()
-    rec = a.forward_record(args.steps, args.checkpoint)
+    rec = a.forward_record(steps, checkpoint)
     t_rec = time.perf_counter() - t0
     t0 = time.perf_counter()
     grads = a.adjoint(rec)
     t_adj = time.perf_counter() - t0
     ok = all(np.isfinite(grads[k]).all() for k in ("gmu", "glam", "gb"))
     a.close()
+    torch.cuda.empty_cache()
+    log(f"forward {t_fwd:.3f}s, forward_record {t_rec:.3f}s, adjoint {t_adj:.3f}s, "
+        f"ratio {(t_rec + t_adj) / t_fwd:.2f}")
+    return m, t_fwd, t_rec, t_adj, ok
+
+
+def bench_adjoint(args):
+    """forward_record + adjoint vs plain forward steps, FP32, same workload."""
+    m, t_fwd, t_rec, t_adj, ok = measure_adjoint(args.steps, args.warmup, args.checkpoint)
     ratio = (t_rec + t_adj) / t_fwd
-    log(f"forward {t_fwd:.3f}s, forward_record {t_rec:.3f}s, adjoint {t_adj:.3f}s, ratio {ratio:.2f}")
     out = {
         "metric": METRIC, "value": m.cells * args.steps / (t_rec + t_adj) / 1e9, "unit": "Gcell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
