@@ -441,7 +441,8 @@ def bench_aweno(args):
                                f"{len(m.stations)} stations + PGV, TVD-RK3", "cells": m.cells, "dt": m.dt},
         "l2_err_fp32_vs_fp64": l2,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "aw_faces",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles("aw_faces_512x512x256_traffic.json"), "kernel": "aw_faces",
                      "launch_ms": launch_ms, "update_launch_ms": upd_ms,
                      "alg_bytes_per_cell": AW_FACE_BYTES, "cells_per_launch": face_cells,
                      "note": "aw_faces is VALU-bound (18 WENO5 interpolations per face); the HBM fraction is "
