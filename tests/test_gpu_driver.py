@@ -146,3 +146,67 @@ def test_driver_outputs_match_oracle(scheme, tmp_path, oracle_mod):
     # model slices written at setup
     vs, nv = gather(str(out / "Vs_Z_mpi_{X}_{Y}_{Z}.bin"), sx, sy, ranks, (NY_, NX_), "X", "Y")
     assert nv == 2 and np.array_equal(vs, m.medium[0, H + 14, H:-H, H:-H])
+
+
+def test_driver_fault_source_matches_oracle(tmp_path, oracle_mod):
+    """useMultiSource: a synthetic fault file (own projection, srcCoord2Index, moment
+    rates) through the driver against the oracle stepping the driver's own table."""
+    lib = T.host_lib()
+    p, st = setup_case(tmp_path, "cgfdm")
+    lon0, lat0 = 101.246376, 37.7942405
+    off = [(0.0, 0.0, -800.0), (0.004, -0.003, -1200.0), (-0.006, 0.002, -1500.0), (0.0, 0.0, -800.0)]
+    nt_src = 60
+    t = np.arange(nt_src) * 0.01
+    rate = np.stack([np.exp(-((t - 0.15 - 0.02 * q) / 0.05) ** 2) for q in range(len(off))])
+    rake = np.full((len(off), nt_src), 30.0)
+    os.makedirs(tmp_path / "src", exist_ok=True)
+    T.write_fault(tmp_path / "src" / "f.bin", [lon0 + o[0] for o in off], [lat0 + o[1] for o in off],
+                  [o[2] for o in off], np.full(len(off), 1e6), np.full(len(off), 40.0), np.full(len(off), 70.0),
+                  rake, rate, 0.01)
+    p.update(useMultiSource=1, sourceDir=str(tmp_path / "src"), sourceFile="f.bin", degree2radian=1,
+             centerLongitude=lon0, centerLatitude=lat0)
+    p["useSingleSource(ricker)"] = 0
+    (tmp_path / "params.json").write_text(json.dumps(p))
+    run_driver(tmp_path, "cgfdm")
+    out = tmp_path / "out"
+    nt = int(0.3 / float(np.float32(0.008)))
+    rc, coord, terrain, medium = T.model(lib, str(tmp_path / "params.json"), NX_, NY_, NZ_)
+    assert rc == 0
+    cells = np.zeros(16, np.int64)
+    rt = np.zeros(16 * nt_src * 6, np.float32)
+    xyz = np.zeros(3 * len(off), np.float32)
+    n, ntv, dtv = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    rc = lib.h_fault_source(str(tmp_path / "params.json").encode(), T.P(cells, ctypes.c_longlong),
+                            T.P(rt, ctypes.c_float), ctypes.byref(n), ctypes.byref(ntv), ctypes.byref(dtv),
+                            T.P(xyz, ctypes.c_float), 16)
+    assert rc == 0 and n.value == len(off)
+    m = M.Model(nx=NX_, ny=NY_, nz=NZ_, dh=100.0, dt=float(np.float32(0.008)), nt=nt, npml=5, coord=coord,
+                medium=medium, free_surface=True)
+    m.src_cells = cells[:n.value]
+    m.src_rate = rt[:n.value * nt_src * 6].reshape(n.value, nt_src, 6)
+    m.src_dt = dtv.value
+    m.stations = (st + H).astype(np.int32)
+    o = oracle_mod.Oracle(m, prec=8, threads=8)
+    o.step(nt)
+    so = o.stations()
+    got = {}
+    for (X, Y, Z) in [(X, Y, Z) for X in range(PXYZ[0]) for Y in range(PXYZ[1]) for Z in range(PXYZ[2])]:
+        f = out / f"station_mpi_{X}_{Y}_{Z}.bin"
+        if not f.exists():
+            continue
+        raw = f.read_bytes()
+        k = len(raw) // (12 + 4 * nt * 9)
+        xyzs = np.frombuffer(raw[:12 * k], np.int32).reshape(k, 3)
+        wave = np.frombuffer(raw[12 * k:], np.float32).reshape(k, nt, 9)
+        for q in range(k):
+            got[tuple(xyzs[q])] = wave[q]
+    assert len(got) == len(st)
+    for s_, (i, j, k) in enumerate(st):
+        for fam in (slice(0, 3), slice(3, 9)):
+            assert rel_l2(got[(i, j, k)][:, fam], so[s_, :, fam]) < 1e-4, (s_, fam)
+    # the source files of the reference layout: one merged cell fewer than the points
+    nfiles = [f for f in os.listdir(out) if f.startswith("source_mpi_")]
+    total = 0
+    for f in nfiles:
+        total += int(np.frombuffer((out / f).read_bytes()[:4], np.int32)[0])
+    assert total == len(set(cells[:n.value].tolist())) == len(off) - 1
