@@ -91,12 +91,7 @@ __global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restri
   if (i >= H + a.nx || j >= H + a.ny) return;
   const long long N = a.N, S = a.S;
   const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
-  // coefficients, products Z and gradient accumulators are streams of this pass
-  // (non-temporal, as in the forward stage kernel)
-  auto cf = [&](int c) -> R {
-    if constexpr (sizeof(TM) == 4) return R(ld_coef(coef + (long long)c * N + p));
-    else return R(Store<TM>::ld(coef + (long long)c * N + p));
-  };
+  auto cf = [&](int c) -> R { return R(Store<TM>::ld(coef + (long long)c * N + p)); };
   const R mu = cf(M_MU), lam = cf(M_LAM), bu = cf(M_B);
   const PmlArgs<R>& P = a.pml;
   const bool top = a.fs && k >= H + a.nz - 3;
@@ -185,11 +180,11 @@ __global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restri
       }
     }
 #pragma unroll
-    for (int v = 0; v < 9; ++v) st_stream(a.Z + (long long)(9 * ax + v) * N + p, Zd[v]);
+    for (int v = 0; v < 9; ++v) a.Z[(long long)(9 * ax + v) * N + p] = Zd[v];
   }
-  st_stream(a.gmu + p, ld_stream(a.gmu + p) + gmu);
-  st_stream(a.glam + p, ld_stream(a.glam + p) + glam);
-  st_stream(a.gb + p, ld_stream(a.gb + p) + gb);
+  a.gmu[p] += gmu;
+  a.glam[p] += glam;
+  a.gb[p] += gb;
 }
 
 // ---------------------------------------------------------------------------
