@@ -50,8 +50,8 @@ def setup_case(tmp_path, scheme):
     return p, np.array(list(st.values()), np.int32)
 
 
-def run_driver(tmp_path, scheme):
-    cmd = [T.RUNNER, "--quiet", "--scheme", scheme]
+def run_driver(tmp_path, scheme, extra=()):
+    cmd = [T.RUNNER, "--quiet", "--scheme", scheme] + list(extra)
     print("driver:", " ".join(cmd), "in", tmp_path, flush=True)
     r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
@@ -210,3 +210,36 @@ def test_driver_fault_source_matches_oracle(tmp_path, oracle_mod):
     for f in nfiles:
         total += int(np.frombuffer((out / f).read_bytes()[:4], np.int32)[0])
     assert total == len(set(cells[:n.value].tolist())) == len(off) - 1
+
+
+@pytest.mark.parametrize("extra,tol", [(("--precision", "8"), 1e-9), (("--coef", "1"), 5e-3)])
+def test_driver_precision_options(extra, tol, tmp_path, oracle_mod):
+    """--precision 8 (FP64 wavefield) and --coef 1 (FP16 coefficient storage) through the
+    driver: station files against the FP64 oracle on the same inputs."""
+    lib = T.host_lib()
+    p, st = setup_case(tmp_path, "cgfdm")
+    nt = int(0.3 / float(np.float32(0.008)))
+    run_driver(tmp_path, "cgfdm", extra)
+    m, cls = oracle_model(lib, tmp_path, "cgfdm", oracle_mod, nt)
+    m.stations = (st + H).astype(np.int32)
+    o = cls(m, prec=8, threads=8)
+    o.step(nt)
+    so = o.stations()
+    out = tmp_path / "out"
+    got = {}
+    for X in range(PXYZ[0]):
+        for Y in range(PXYZ[1]):
+            for Z in range(PXYZ[2]):
+                f = out / f"station_mpi_{X}_{Y}_{Z}.bin"
+                if not f.exists():
+                    continue
+                raw = f.read_bytes()
+                k = len(raw) // (12 + 4 * nt * 9)
+                xyz = np.frombuffer(raw[:12 * k], np.int32).reshape(k, 3)
+                wave = np.frombuffer(raw[12 * k:], np.float32).reshape(k, nt, 9)
+                for q in range(k):
+                    got[tuple(xyz[q])] = wave[q]
+    for s_, (i, j, k) in enumerate(st):
+        for fam in (slice(0, 3), slice(3, 9)):
+            # the files hold float32 samples: the FP64 run agrees to float rounding
+            assert rel_l2(got[(i, j, k)][:, fam], so[s_, :, fam]) < max(tol, 1e-6), (extra, s_, fam)
