@@ -42,6 +42,10 @@ CASES = {
     "no_free_surface": dict(nx=22, ny=18, nz=14, npml=4, hills=0, layered=True, free_surface=False),
     "no_pml_flat": dict(nx=20, ny=16, nz=12, npml=0, hills=0, layered=False, free_surface=True),
     "two_x_tiles": dict(nx=70, ny=12, nz=14, npml=5, hills=2, layered=True, free_surface=True),
+    # three 64-column x tiles, the middle one clear of both x PML slabs, with and
+    # without the free surface (PML on the top face then)
+    "three_x_tiles": dict(nx=140, ny=16, nz=12, npml=4, hills=2, layered=True, free_surface=True),
+    "three_x_tiles_z2": dict(nx=140, ny=16, nz=14, npml=4, hills=0, layered=True, free_surface=False),
 }
 
 
