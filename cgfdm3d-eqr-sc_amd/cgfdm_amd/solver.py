@@ -41,7 +41,7 @@ EXPORTS = ["cgfdm_create", "cgfdm_destroy", "cgfdm_set_source", "cgfdm_set_stati
            "cgfdm_set_stream", "cgfdm_synchronize", "cgfdm_step_index", "cgfdm_get_stations",
            "cgfdm_get_pgv", "cgfdm_get_surface_displacement", "cgfdm_get_field", "cgfdm_set_field",
            "cgfdm_get_setup", "cgfdm_get_ranges", "cgfdm_timing", "cgfdm_device_bytes",
-           "cgfdm_last_error", "cgfdm_grid_ranges", "cgfdm_forward_record", "cgfdm_adjoint"]
+           "cgfdm_last_error", "cgfdm_grid_ranges", "cgfdm_forward_record", "cgfdm_adjoint", "cgfdm_get_slice"]
 
 
 def lib():
@@ -65,6 +65,7 @@ def lib():
     L.cgfdm_get_pgv.argtypes = [vp, vp]
     L.cgfdm_get_surface_displacement.argtypes = [vp, vp]
     L.cgfdm_get_field.argtypes = [vp, i, vp]
+    L.cgfdm_get_slice.argtypes = [vp, i, i, i, vp]
     L.cgfdm_set_field.argtypes = [vp, i, vp]
     L.cgfdm_get_setup.argtypes = [vp, i, vp]
     L.cgfdm_get_ranges.argtypes = [vp, vp]
@@ -187,6 +188,16 @@ class Solver:
 
     def wavefield(self):
         return np.stack([self.field(c) for c in range(9)])
+
+    def slice(self, comp, axis, index):
+        """One component on an interior plane at padded `index`, float32 in physical
+        units (Vx..Tyz for both schemes): axis 0 -> [nz][ny], 1 -> [nz][nx], 2 -> [ny][nx]."""
+        NZ, NY, NX = self.shape
+        nz, ny, nx = NZ - 6, NY - 6, NX - 6
+        shp = {0: (nz, ny), 1: (nz, nx), 2: (ny, nx)}[int(axis)]
+        out = np.empty(shp, np.float32)
+        _check(self.L.cgfdm_get_slice(self.h, int(comp), int(axis), int(index), _ptr(out)))
+        return out
 
     def set_field(self, comp, arr):
         a = np.ascontiguousarray(arr, np.float64).ravel()
