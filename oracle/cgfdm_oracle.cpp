@@ -814,6 +814,35 @@ void orc_get_face(void* hv, int f, int comp, double* out) {
     for (long long q = 0; q < F.cnt; ++q) out[q] = (double)F.a[comp * F.cnt + q];
   });
 }
+// Right-hand side h_Aux of face f (filled by orc_rhs with boundaries)
+void orc_get_face_rhs(void* hv, int f, int comp, double* out) {
+  auto* h = (OrcHandle*)hv;
+  DISPATCH(h, {
+    auto& F = o->faces[f];
+    for (long long q = 0; q < F.cnt; ++q) out[q] = (double)F.h[comp * F.cnt + q];
+  });
+}
+void orc_set_face(void* hv, int f, int comp, const double* in) {
+  auto* h = (OrcHandle*)hv;
+  DISPATCH(h, {
+    auto& F = o->faces[f];
+    using T = std::remove_reference<decltype(F.a[0])>::type;
+    for (long long q = 0; q < F.cnt; ++q) F.a[comp * F.cnt + q] = (T)in[q];
+  });
+}
+// Axis (0..2) and side (0 low, 1 high) of face f
+void orc_face_info(void* hv, int f, int* axis_side) {
+  auto* h = (OrcHandle*)hv;
+  DISPATCH(h, {
+    axis_side[0] = o->faces[f].axis;
+    axis_side[1] = o->faces[f].side;
+  });
+}
+// Source injection at the current step index, without stepping
+void orc_source_only(void* hv) {
+  auto* h = (OrcHandle*)hv;
+  DISPATCH(h, o->add_source());
+}
 
 // Single operator applications for the closed-form tests: fills h_W with the
 // interior RHS (+ free surface + PML terms) for the current W without

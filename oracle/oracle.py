@@ -12,6 +12,7 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
+H = 3  # ghost layers on every side (reference macro.h:39)
 _LIB = None
 
 
@@ -48,6 +49,10 @@ def lib():
         L.orc_face_count.restype = ll
         L.orc_face_count.argtypes = [vp, i]
         L.orc_get_face.argtypes = [vp, i, i, vp]
+        L.orc_get_face_rhs.argtypes = [vp, i, i, vp]
+        L.orc_set_face.argtypes = [vp, i, i, vp]
+        L.orc_face_info.argtypes = [vp, i, vp]
+        L.orc_source_only.argtypes = [vp]
         L.orc_rhs.argtypes = [vp, i, i, i, i]
         L.orc_rk_stage.argtypes = [i, i, ll, vp, vp, vp, vp]
         L.orc_step_index.restype = i
@@ -190,6 +195,49 @@ class Oracle:
         out = np.empty(n, np.float64)
         self.L.orc_get_face(self.h, f, comp, _ptr(out))
         return out
+
+    def face_rhs(self, f, comp):
+        """h_Aux of face f, component comp (after rhs(..., with_boundaries=True))."""
+        n = self.L.orc_face_count(self.h, f)
+        out = np.empty(n, np.float64)
+        self.L.orc_get_face_rhs(self.h, f, comp, _ptr(out))
+        return out
+
+    def set_face(self, f, comp, arr):
+        a = np.ascontiguousarray(arr, np.float64).ravel()
+        assert a.size == self.L.orc_face_count(self.h, f)
+        self.L.orc_set_face(self.h, f, comp, _ptr(a))
+
+    def face_info(self, f):
+        """(axis, side) of face f in launch order."""
+        out = np.zeros(2, np.int32)
+        self.L.orc_face_info(self.h, f, _ptr(out))
+        return int(out[0]), int(out[1])
+
+    def face_cells(self, f):
+        """Padded (i, j, k) of every entry of face f's slab, in slab order."""
+        ax, side = self.face_info(f)
+        m = self.m
+        n0, n1 = {0: (m.npml, m.ny), 1: (m.nx, m.npml), 2: (m.nx, m.ny)}[ax]
+        q = np.arange(self.L.orc_face_count(self.h, f))
+        i = q % n0 + H
+        j = (q // n0) % n1 + H
+        k = q // (n0 * n1) + H
+        if side:
+            if ax == 0:
+                i = i + m.nx - m.npml
+            elif ax == 1:
+                j = j + m.ny - m.npml
+            else:
+                k = k + m.nz - m.npml
+        return i, j, k
+
+    def source_only(self):
+        """Inject the source at the current step index without stepping."""
+        self.L.orc_source_only(self.h)
+
+    def set_step_index(self, it):
+        self.L.orc_set_step_index(self.h, int(it))
 
 
 def adjoint_dot(o, which, seed=1):
