@@ -32,7 +32,7 @@ struct AdjArgs {
 };
 
 template <class R, class TM>
-void launch_adj_stage(AdjArgs<R> a, const TM* coef, const R* jac, const R* rdzdx, const R* rdzdy, int fb1, int fb2,
+int launch_adj_stage(AdjArgs<R> a, const TM* coef, const R* jac, const R* rdzdx, const R* rdzdy, int fb1, int fb2,
                       int fb3, hipStream_t st);
 template <class R>
 void launch_adj_inject(R* mu, long long N, int ns, const long long* cell, const R* src, int nsteps, int n, hipStream_t st);

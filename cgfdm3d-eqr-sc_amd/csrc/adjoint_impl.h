@@ -609,9 +609,11 @@ void adj_stage_fb(const AdjArgs<R>& a, const TM* coef, const R* jac, const R* rd
 }
 
 template <class R, class TM>
-void launch_adj_stage(AdjArgs<R> a, const TM* coef, const R* jac, const R* rdzdx, const R* rdzdy, int fb1, int fb2,
-                      int fb3, hipStream_t st) {
-  if (a.stage < 0 || a.stage > 3) return;
+int launch_adj_stage(AdjArgs<R> a, const TM* coef, const R* jac, const R* rdzdx, const R* rdzdy, int fb1, int fb2,
+                     int fb3, hipStream_t st) {
+  if (a.stage < 0 || a.stage > 3) return LAUNCH_BAD_STAGE;
+  if (a.nx <= 0 || a.ny <= 0 || a.nz <= 0) return LAUNCH_EMPTY;
+  if ((a.ny + 3) / 4 > 65535 || a.nz > 65535) return LAUNCH_TOO_LARGE;
   const dim3 grid((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz), block(256);
   const dim3 gs((a.nx + 63) / 64, (a.ny + 3) / 4, 3);
   // the surface pass reads the auxiliary adjoint g of this stage, which pass A
@@ -626,6 +628,7 @@ void launch_adj_stage(AdjArgs<R> a, const TM* coef, const R* jac, const R* rdzdx
     case 6: adj_stage_fb<R, TM, -1, 1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
     default: adj_stage_fb<R, TM, 1, 1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
   }
+  return LAUNCH_OK;
 }
 
 template <class R>

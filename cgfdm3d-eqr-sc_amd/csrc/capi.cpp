@@ -42,6 +42,17 @@ const char* const kCgfdmStepOff = "the CGFDM step kernels are switched off in th
     if (e_ != hipSuccess) return fail(CGFDM_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// A stage launcher that refuses its launch (invalid stage, grid beyond the
+// launch limits) is a caller error: the step fails instead of leaving the
+// wavefield one stage behind.
+#define LAUNCHCHK(x, what)                                                                         \
+  do {                                                                                             \
+    const int r_ = (x);                                                                            \
+    if (r_ == cg::LAUNCH_BAD_STAGE) return fail(CGFDM_ERR_ARG, std::string(what) + ": RK stage out of range"); \
+    if (r_ == cg::LAUNCH_TOO_LARGE)                                                                \
+      return fail(CGFDM_ERR_ARG, std::string(what) + ": grid beyond the kernel launch limits");     \
+  } while (0)
+
 // F/B schedule of the MacCormack operators, selected by it % 8 and flipped
 // after every RK stage (reference propagate.cpp:158-168, 195-197, 248-250).
 const int FBTAB[8][3] = {{-1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, -1, 1},
@@ -323,6 +334,7 @@ struct Solver : SolverBase {
   // interior block order: regions of 64 x (4 rty) columns per XCD; the
   // environment variable CGFDM_INTERIOR_RTY (1..64) overrides it for tuning
   int rty = 4;
+  int rtx = 1;
   // interior kernel choice (FP32 wavefield): 0 thread-per-cell, -1 x-paired
   // (FP32 coefficients, even NX), 1 plane tile, > 1 z-marching with chunks of kc
   // planes. CGFDM_INTERIOR_KC (-1..4096) overrides it.
@@ -335,12 +347,19 @@ struct Solver : SolverBase {
       const int v = std::atoi(e);
       if (v >= 1 && v <= 64) rty = v;
     }
+    if (const char* e = std::getenv("CGFDM_INTERIOR_RTX")) {
+      const int v = std::atoi(e);
+      if (v >= 1 && v <= 64) rtx = v;
+    }
     if (const char* e = std::getenv("CGFDM_INTERIOR_KC")) {
       const int v = std::atoi(e);
       if (v >= -1 && v <= 4096) kc = v;
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
     if (const char* e = std::getenv("CGFDM_ROW_ALIGN")) galign = std::atoi(e) != 0;
+    // the x-paired kernel pairs cells (i, i + 1) with even i on 8-byte
+    // boundaries, which the tight layout gives and the row-aligned one does not
+    if (kc == -1) galign = false;
     // setup runs on the tight layout; its products are repacked into the
     // row-aligned layout below
     tNX = d.nx + 2 * cg::H;
@@ -573,7 +592,7 @@ struct Solver : SolverBase {
       size_t sp = 0;
       if (timing_on) {
         sp = next_pair(2);
-        hipEventRecord(ev[sp].a, stream);
+        HIPCHK(hipEventRecord(ev[sp].a, stream));
       }
       // moment-rate source on W at t = it * DT (addMoment.cpp:262-377), float
       // time arithmetic as in the reference
@@ -600,6 +619,7 @@ struct Solver : SolverBase {
         a.kend = kend;
         a.stage = irk;
         a.rty = rty;
+        a.rtx = rtx;
         a.kc = kc;
         a.zty = zty;
         a.DT = R(d.dt);
@@ -612,26 +632,26 @@ struct Solver : SolverBase {
         size_t ip = 0;
         if (timing_on) {
           ip = next_pair(0);
-          hipEventRecord(ev[ip].a, stream);
+          HIPCHK(hipEventRecord(ev[ip].a, stream));
         }
-        cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream);
-        if (timing_on) hipEventRecord(ev[ip].b, stream);
+        LAUNCHCHK((cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream)), "interior stage");
+        if (timing_on) HIPCHK(hipEventRecord(ev[ip].b, stream));
         if (d.free_surface) {
           cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, observe ? pgv : nullptr, dis};
           size_t fp = 0;
           if (timing_on) {
             fp = next_pair(1);
-            hipEventRecord(ev[fp].a, stream);
+            HIPCHK(hipEventRecord(ev[fp].a, stream));
           }
-          cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], stream);
-          if (timing_on) hipEventRecord(ev[fp].b, stream);
+          LAUNCHCHK((cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], stream)), "surface stage");
+          if (timing_on) HIPCHK(hipEventRecord(ev[fp].b, stream));
         }
         fb[0] = -fb[0];
         fb[1] = -fb[1];
         fb[2] = -fb[2];
       }
       if (observe) cg::launch_record<R>(A, N, ns, st_cell, st_rec, d.nt, it, stream);
-      if (timing_on) hipEventRecord(ev[sp].b, stream);
+      if (timing_on) HIPCHK(hipEventRecord(ev[sp].b, stream));
       ++it;
     }
     HIPCHK(hipGetLastError());
@@ -785,13 +805,13 @@ struct Solver : SolverBase {
     for (int irk = 0; irk < 4; ++irk) {
       cg::StageArgs<R> a{};
       a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.NX = NX; a.NY = NY; a.N = N; a.S = S;
-      a.kbeg = cg::H; a.kend = kend; a.stage = irk; a.rty = rty; a.kc = kc; a.zty = zty;
+      a.kbeg = cg::H; a.kend = kend; a.stage = irk; a.rty = rty; a.rtx = rtx; a.kc = kc; a.zty = zty;
       a.DT = R(d.dt); a.rDH = R(1.0 / d.dh);
       a.win = ins[irk]; a.wout = outs[irk]; a.wm = A; a.wt = T; a.pml = pml;
-      cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream);
+      LAUNCHCHK((cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream)), "interior stage");
       if (d.free_surface) {
         cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, nullptr, dis};
-        cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], stream);
+        LAUNCHCHK((cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], stream)), "surface stage");
       }
       fb[0] = -fb[0]; fb[1] = -fb[1]; fb[2] = -fb[2];
     }
@@ -813,7 +833,7 @@ struct Solver : SolverBase {
   // RK stage states of the step at index `step` starting from A (after the
   // source): X1 -> B, X2 -> C, X3 -> x3. Auxiliary states evolve in the face
   // stage buffers; the face state buffers and A are left unchanged.
-  void stage_states(int step, R* x3) {
+  int stage_states(int step, R* x3) {
     const int kend = cg::H + d.nz - (d.free_surface ? 3 : 0);
     int fb[3] = {FBTAB[step % 8][0], FBTAB[step % 8][1], FBTAB[step % 8][2]};
     R* ins[3] = {A, B, C};
@@ -821,16 +841,17 @@ struct Solver : SolverBase {
     for (int irk = 0; irk < 3; ++irk) {
       cg::StageArgs<R> a{};
       a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.NX = NX; a.NY = NY; a.N = N; a.S = S;
-      a.kbeg = cg::H; a.kend = kend; a.stage = irk; a.rty = rty; a.kc = kc; a.zty = zty;
+      a.kbeg = cg::H; a.kend = kend; a.stage = irk; a.rty = rty; a.rtx = rtx; a.kc = kc; a.zty = zty;
       a.DT = R(d.dt); a.rDH = R(1.0 / d.dh);
       a.win = ins[irk]; a.wout = outs[irk]; a.wm = A; a.wt = T; a.pml = pml;
-      cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream);
+      LAUNCHCHK((cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream)), "interior stage");
       if (d.free_surface) {
         cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, nullptr, dis};
-        cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], stream);
+        LAUNCHCHK((cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], stream)), "surface stage");
       }
       fb[0] = -fb[0]; fb[1] = -fb[1]; fb[2] = -fb[2];
     }
+    return CGFDM_OK;
   }
 
   int forward_record(int nsteps, int ck, double* rec) override {
@@ -904,10 +925,25 @@ struct Solver : SolverBase {
       }
     }
     if (!adj_stages) {
-      double gib = 64.0;
-      if (const char* e = std::getenv("CGFDM_ADJOINT_STORE_GB")) gib = std::atof(e);
+      // budget of the stage store: CGFDM_ADJOINT_STORE_GB, else 85 % of the
+      // device memory still free (288 GB on an MI355X hold the stage states of a
+      // 100-step segment of a 256 x 256 x 128 grid)
+      double budget = 0.0;
+      if (const char* e = std::getenv("CGFDM_ADJOINT_STORE_GB")) {
+        budget = std::atof(e) * 1073741824.0;
+      } else {
+        size_t fre = 0, tot = 0;
+        if (hipMemGetInfo(&fre, &tot) == hipSuccess) budget = 0.85 * (double)fre;
+        (void)hipGetLastError();
+      }
       const double need = (double)sizeof(R) * 36.0 * (double)N * fr_ck;
-      if (need <= gib * 1073741824.0 && dalloc(&adj_stages, 36LL * N * fr_ck) != CGFDM_OK) adj_stages = nullptr;
+      if (need <= budget && dalloc(&adj_stages, 36LL * N * fr_ck) != CGFDM_OK) {
+        // the stage store is optional: fall back to recomputing the stages, and
+        // clear the runtime's sticky error of the failed allocation so that the
+        // launch checks of the recompute path do not report it
+        adj_stages = nullptr;
+        (void)hipGetLastError();
+      }
     }
     if ((rc = copy_state(adj_end, true))) return rc;
     const int nck = (fr_n + fr_ck - 1) / fr_ck;
@@ -937,7 +973,7 @@ struct Solver : SolverBase {
         } else {
           if ((rc = copy_state(adj_seg + (long long)(n - n0) * ss, false))) return rc;
           srcIt = inject_source(step, tw);
-          stage_states(step, X3);
+          if ((rc = stage_states(step, X3))) return rc;
           X[0] = A; X[1] = B; X[2] = C; X[3] = X3;
         }
         const int* fb0 = FBTAB[step % 8];
@@ -955,7 +991,7 @@ struct Solver : SolverBase {
           a.pml = pml;
           for (int f = 0; f < 6; ++f) a.fa[f] = fa[f];
           if constexpr (std::is_same<R, TM>::value)
-            cg::launch_adj_stage<R, TM>(a, coef, jac, rdzdx, rdzdy, sg * fb0[0], sg * fb0[1], sg * fb0[2], stream);
+            LAUNCHCHK((cg::launch_adj_stage<R, TM>(a, coef, jac, rdzdx, rdzdy, sg * fb0[0], sg * fb0[1], sg * fb0[2], stream)), "adjoint stage");
         }
         if (srcIt >= 0)
           cg::launch_adj_source<R>(OUT, N, src_np, src_cell, adj_gsrc, srcIt, (double)tw, (double)R(d.dt), stream);
@@ -1033,6 +1069,7 @@ struct AwenoSolver : SolverBase {
   // interior block order: regions of 64 x (4 rty) columns per XCD; the
   // environment variable CGFDM_INTERIOR_RTY (1..64) overrides it for tuning
   int rty = 4;
+  int rtx = 1;
   // interior kernel choice (FP32 wavefield): 0 thread-per-cell, -1 x-paired
   // (FP32 coefficients, even NX), 1 plane tile, > 1 z-marching with chunks of kc
   // planes. CGFDM_INTERIOR_KC (-1..4096) overrides it.
@@ -1172,7 +1209,7 @@ struct AwenoSolver : SolverBase {
       size_t sp = 0;
       if (timing_on) {
         sp = next_pair(2);
-        hipEventRecord(ev[sp].a, stream);
+        HIPCHK(hipEventRecord(ev[sp].a, stream));
       }
       R* W = buf[cur];
       if (src_np > 0) {
@@ -1208,16 +1245,16 @@ struct AwenoSolver : SolverBase {
         size_t fp = 0, up = 0;
         if (timing_on) {
           fp = next_pair(0);
-          hipEventRecord(ev[fp].a, stream);
+          HIPCHK(hipEventRecord(ev[fp].a, stream));
         }
         aw::launch_faces<R>(a, stream);
         if (timing_on) {
-          hipEventRecord(ev[fp].b, stream);
+          HIPCHK(hipEventRecord(ev[fp].b, stream));
           up = next_pair(1);
-          hipEventRecord(ev[up].a, stream);
+          HIPCHK(hipEventRecord(ev[up].a, stream));
         }
         aw::launch_update<R>(a, d.nx, d.ny, stream);
-        if (timing_on) hipEventRecord(ev[up].b, stream);
+        if (timing_on) HIPCHK(hipEventRecord(ev[up].b, stream));
       }
       cur = o1;
       W = buf[cur];
@@ -1225,7 +1262,7 @@ struct AwenoSolver : SolverBase {
       aw::launch_sponge<R>(W, d.nx, d.ny, d.nz, NX, N, S, d.free_surface ? 1 : 0, stream);
       if (d.free_surface) aw::launch_surface_obs<R>(W, cjm, pgv, dis, d.nx, d.ny, NX, NZ, N, S, R(d.dt), stream);
       aw::launch_record<R>(W, cjm, N, ns, st_cell, st_rec, d.nt, it, stream);
-      if (timing_on) hipEventRecord(ev[sp].b, stream);
+      if (timing_on) HIPCHK(hipEventRecord(ev[sp].b, stream));
       ++it;
     }
     HIPCHK(hipGetLastError());
@@ -1440,6 +1477,12 @@ int cgfdm_step(cgfdm_solver* s, int nsteps) {
 
 int cgfdm_set_stream(cgfdm_solver* s, void* stream) {
   SOLVER_OR_FAIL(s);
+  // work queued on the old stream finishes before anything is queued on the
+  // new one, so the switch keeps the one-stream order of the handle
+  if ((hipStream_t)stream != s->impl->stream) {
+    hipError_t e = hipStreamSynchronize(s->impl->stream);
+    if (e != hipSuccess) return fail(CGFDM_ERR_HIP, hipGetErrorString(e));
+  }
   s->impl->stream = (hipStream_t)stream;
   return CGFDM_OK;
 }

@@ -212,41 +212,44 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
 
 // XCD-aware block order of the interior launch. The hardware hands block b to
 // XCD b % 8, so block b is the (b / 8)-th block that XCD runs. The (x, y) plane
-// is cut into regions of one 64-column tile by rty 4-row tiles; regions are dealt
-// to the XCDs in turn and each XCD walks its regions one after another, plane by
-// plane in k and tile by tile within a plane. The z and y stencil neighbours of a
-// cell are then read by blocks of the same XCD a few planes earlier or later, and
-// come from that XCD's L2 instead of HBM. Returns false for padding blocks.
+// is cut into regions of rtx 64-column tiles by rty 4-row tiles; regions are
+// dealt to the XCDs in turn and each XCD walks its regions one after another,
+// plane by plane in k and, within a plane, row of tiles by row of tiles. The
+// x, y and z stencil neighbours of a cell are then read by blocks of the same
+// XCD a few planes earlier or later and come from that XCD's L2 instead of
+// HBM, except across region edges. Returns false for padding blocks.
 struct CellTile {
   int i0, j0, k;
 };
-__device__ __forceinline__ bool interior_tile(int b, int nx, int ny, int nk, int rty, CellTile& t) {
+__device__ __forceinline__ bool interior_tile(int b, int nx, int ny, int nk, int rtx, int rty, CellTile& t) {
   const int ntx = (nx + 63) / 64, nty = (ny + 3) / 4;
-  const int nry = (nty + rty - 1) / rty;
-  const int per_region = rty * nk;
+  const int nrx = (ntx + rtx - 1) / rtx, nry = (nty + rty - 1) / rty;
+  const int per_plane = rtx * rty;
+  const int per_region = per_plane * nk;
   const int xcd = b & 7, seq = b >> 3;
   const int slot = seq / per_region, within = seq - slot * per_region;
   const int r = slot * 8 + xcd;
-  const int kk = within / rty, ty = within - kk * rty;
-  const int rx = r % ntx, ry = r / ntx;
-  const int tyg = ry * rty + ty;
-  t.i0 = rx * 64;
+  const int kk = within / per_plane, q = within - kk * per_plane;
+  const int ty = q / rtx, tx = q - ty * rtx;
+  const int rx = r % nrx, ry = r / nrx;
+  const int txg = rx * rtx + tx, tyg = ry * rty + ty;
+  t.i0 = txg * 64;
   t.j0 = tyg * 4;
   t.k = kk;
-  return r < ntx * nry && tyg < nty;
+  return r < nrx * nry && txg < ntx && tyg < nty;
 }
 
 // number of blocks of the interior launch for interior_tile()
-inline long long interior_blocks(int nx, int ny, int nk, int rty) {
+inline long long interior_blocks(int nx, int ny, int nk, int rtx, int rty) {
   const long long ntx = (nx + 63) / 64, nty = (ny + 3) / 4;
-  const long long nregions = ntx * ((nty + rty - 1) / rty);
-  return 8 * ((nregions + 7) / 8) * (long long)rty * nk;
+  const long long nregions = ((ntx + rtx - 1) / rtx) * ((nty + rty - 1) / rty);
+  return 8 * ((nregions + 7) / 8) * (long long)rtx * rty * nk;
 }
 
 template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
 __global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restrict__ coef) {
   CellTile t;
-  if (!interior_tile(blockIdx.x, a.nx, a.ny, a.kend - a.kbeg, a.rty, t)) return;
+  if (!interior_tile(blockIdx.x, a.nx, a.ny, a.kend - a.kbeg, a.rtx, a.rty, t)) return;
   const int i = H + t.i0 + (threadIdx.x & 63);
   const int j = H + t.j0 + (threadIdx.x >> 6);
   const int k = a.kbeg + t.k;

@@ -12,7 +12,7 @@ struct StageArgs {
   long long S;          // NX * NY
   int kbeg, kend;       // k range of the interior kernel [kbeg, kend)
   int stage;            // RK4 stage 0..3
-  int rty;              // interior block order: regions of 64 x (4 rty) columns per XCD
+  int rtx, rty;         // interior block order: regions of (64 rtx) x (4 rty) columns per XCD
   int kc;               // > 0: z-marching interior kernel with chunks of kc planes
   int zty;              // z-marching tile height: 8 (512 threads) or 4 (256 threads)
   R DT, rDH;
@@ -32,10 +32,17 @@ struct SurfArgs {
   R* dis;               // surface displacement, 3 planes [NY][NX]
 };
 
+// launcher results: nothing is launched unless LAUNCH_OK; LAUNCH_EMPTY means
+// there is nothing to do (an empty k range), the others are caller errors
+constexpr int LAUNCH_OK = 0;
+constexpr int LAUNCH_EMPTY = 1;       // empty range: no launch needed
+constexpr int LAUNCH_BAD_STAGE = 2;   // RK stage outside 0..3
+constexpr int LAUNCH_TOO_LARGE = 3;   // grid beyond the launch limits (2^31 - 1 blocks, 65535 in y)
+
 template <class R, class TM>
-void launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, hipStream_t st);
+int launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, hipStream_t st);
 template <class R, class TM>
-void launch_surface(StageArgs<R> a, const TM* coef, SurfArgs<R> s, int fb1, int fb2, int fb3, hipStream_t st);
+int launch_surface(StageArgs<R> a, const TM* coef, SurfArgs<R> s, int fb1, int fb2, int fb3, hipStream_t st);
 template <class R>
 void launch_source(R* W, long long N, int np, const long long* cell, const R* rate, int srcIt, R tw, R DT, hipStream_t st);
 template <class R>

@@ -75,14 +75,17 @@ void surface_stage(const StageArgs<R>& a, const TM* coef, const SurfArgs<R>& s, 
 }
 
 template <class R, class TM>
-void launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, hipStream_t st) {
+int launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, hipStream_t st) {
   const int nk = a.kend - a.kbeg;
-  if (nk <= 0 || a.nx <= 0 || a.ny <= 0 || a.stage < 0 || a.stage > 3) return;
+  if (a.stage < 0 || a.stage > 3) return LAUNCH_BAD_STAGE;
+  if (nk <= 0 || a.nx <= 0 || a.ny <= 0) return LAUNCH_EMPTY;
   if constexpr (std::is_same<R, float>::value && std::is_same<TM, float>::value) {
-    if (a.kc < 0 && a.NX % 2 == 0) {
+    // 8-byte pair accesses need every array's even columns on 8-byte boundaries
+    auto al8 = [](const void* q) { return (reinterpret_cast<unsigned long long>(q) & 7) == 0; };
+    if (a.kc < 0 && a.NX % 2 == 0 && a.N % 2 == 0 && al8(a.win) && al8(a.wout) && al8(a.wm) && al8(a.wt) && al8(coef)) {
       // x-paired kernel: 128 x 4 cells per block, 8-byte accesses
       const long long nb = (long long)((a.nx + 1 + 127) / 128) * ((a.ny + 3) / 4) * nk;
-      if (nb > 0x7fffffffLL) return;
+      if (nb > 0x7fffffffLL) return LAUNCH_TOO_LARGE;
       const dim3 grid((unsigned)nb), block(256);
       switch (fb_index(fb1, fb2, fb3)) {
         case 0: pair_stage<-1, -1, -1>(a, coef, grid, block, st); break;
@@ -94,7 +97,7 @@ void launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, 
         case 6: pair_stage<-1, 1, 1>(a, coef, grid, block, st); break;
         default: pair_stage<1, 1, 1>(a, coef, grid, block, st); break;
       }
-      return;
+      return LAUNCH_OK;
     }
   }
   if constexpr (std::is_same<R, float>::value) {
@@ -102,7 +105,7 @@ void launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, 
     // z-marching kernel (FP32 wavefield): tiles of 64 x zty columns, chunks of kc planes
     if (a.zty != 4) a.zty = 8;
     const long long nb = (long long)((a.nx + 63) / 64) * ((a.ny + a.zty - 1) / a.zty) * ((nk + a.kc - 1) / a.kc);
-    if (nb > 0x7fffffffLL) return;
+    if (nb > 0x7fffffffLL) return LAUNCH_TOO_LARGE;
     const dim3 grid((unsigned)nb), block(64 * a.zty);
     switch (fb_index(fb1, fb2, fb3)) {
       case 0: zcol_stage<R, TM, -1, -1, -1>(a, coef, grid, block, st); break;
@@ -114,12 +117,13 @@ void launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, 
       case 6: zcol_stage<R, TM, -1, 1, 1>(a, coef, grid, block, st); break;
       default: zcol_stage<R, TM, 1, 1, 1>(a, coef, grid, block, st); break;
     }
-    return;
+    return LAUNCH_OK;
    }
   }
   if (a.rty < 1) a.rty = 1;
-  const long long nb = interior_blocks(a.nx, a.ny, nk, a.rty);
-  if (nb > 0x7fffffffLL) return;
+  if (a.rtx < 1) a.rtx = 1;
+  const long long nb = interior_blocks(a.nx, a.ny, nk, a.rtx, a.rty);
+  if (nb > 0x7fffffffLL) return LAUNCH_TOO_LARGE;
   const dim3 grid((unsigned)nb), block(256);
   switch (fb_index(fb1, fb2, fb3)) {
     case 0: cell_stage<R, TM, -1, -1, -1>(a, coef, grid, block, st); break;
@@ -131,11 +135,14 @@ void launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, 
     case 6: cell_stage<R, TM, -1, 1, 1>(a, coef, grid, block, st); break;
     default: cell_stage<R, TM, 1, 1, 1>(a, coef, grid, block, st); break;
   }
+  return LAUNCH_OK;
 }
 
 template <class R, class TM>
-void launch_surface(StageArgs<R> a, const TM* coef, SurfArgs<R> s, int fb1, int fb2, int fb3, hipStream_t st) {
-  if (a.stage < 0 || a.stage > 3) return;
+int launch_surface(StageArgs<R> a, const TM* coef, SurfArgs<R> s, int fb1, int fb2, int fb3, hipStream_t st) {
+  if (a.stage < 0 || a.stage > 3) return LAUNCH_BAD_STAGE;
+  if (a.nx <= 0 || a.ny <= 0) return LAUNCH_EMPTY;
+  if ((a.ny + 3) / 4 > 65535) return LAUNCH_TOO_LARGE;
   const dim3 grid((a.nx + 63) / 64, (a.ny + 3) / 4, 3), block(256);
   switch (fb_index(fb1, fb2, fb3)) {
     case 0: surface_stage<R, TM, -1, -1, -1>(a, coef, s, grid, block, st); break;
@@ -147,6 +154,7 @@ void launch_surface(StageArgs<R> a, const TM* coef, SurfArgs<R> s, int fb1, int 
     case 6: surface_stage<R, TM, -1, 1, 1>(a, coef, s, grid, block, st); break;
     default: surface_stage<R, TM, 1, 1, 1>(a, coef, s, grid, block, st); break;
   }
+  return LAUNCH_OK;
 }
 
 template <class R>

@@ -2,8 +2,8 @@
 #include "launch_impl.h"
 
 namespace cg {
-template void launch_interior<float, float>(StageArgs<float>, const float*, int, int, int, hipStream_t);
-template void launch_surface<float, float>(StageArgs<float>, const float*, SurfArgs<float>, int, int, int, hipStream_t);
+template int launch_interior<float, float>(StageArgs<float>, const float*, int, int, int, hipStream_t);
+template int launch_surface<float, float>(StageArgs<float>, const float*, SurfArgs<float>, int, int, int, hipStream_t);
 }  // namespace cg
 
 namespace cg {

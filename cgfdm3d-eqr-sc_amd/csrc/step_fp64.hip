@@ -2,8 +2,8 @@
 #include "launch_impl.h"
 
 namespace cg {
-template void launch_interior<double, double>(StageArgs<double>, const double*, int, int, int, hipStream_t);
-template void launch_surface<double, double>(StageArgs<double>, const double*, SurfArgs<double>, int, int, int, hipStream_t);
+template int launch_interior<double, double>(StageArgs<double>, const double*, int, int, int, hipStream_t);
+template int launch_surface<double, double>(StageArgs<double>, const double*, SurfArgs<double>, int, int, int, hipStream_t);
 }  // namespace cg
 
 namespace cg {
