@@ -41,7 +41,8 @@ EXPORTS = ["cgfdm_create", "cgfdm_destroy", "cgfdm_set_source", "cgfdm_set_stati
            "cgfdm_set_stream", "cgfdm_synchronize", "cgfdm_step_index", "cgfdm_get_stations",
            "cgfdm_get_pgv", "cgfdm_get_surface_displacement", "cgfdm_get_field", "cgfdm_set_field",
            "cgfdm_get_setup", "cgfdm_get_ranges", "cgfdm_timing", "cgfdm_device_bytes",
-           "cgfdm_last_error", "cgfdm_grid_ranges", "cgfdm_forward_record", "cgfdm_adjoint", "cgfdm_get_slice"]
+           "cgfdm_last_error", "cgfdm_grid_ranges", "cgfdm_forward_record", "cgfdm_adjoint", "cgfdm_get_slice",
+           "cgfdm_grid_ranges_noalloc"]
 
 
 def lib():
@@ -74,6 +75,7 @@ def lib():
     L.cgfdm_device_bytes.argtypes = [vp]
     L.cgfdm_last_error.restype = ctypes.c_char_p
     L.cgfdm_grid_ranges.argtypes = [i, i, i, vp, vp, vp]
+    L.cgfdm_grid_ranges_noalloc.argtypes = [i, i, i, vp, vp, vp, vp]
     L.cgfdm_forward_record.argtypes = [vp, i, i, vp]
     L.cgfdm_adjoint.argtypes = [vp, vp, vp, vp, vp, vp]
     _lib = L

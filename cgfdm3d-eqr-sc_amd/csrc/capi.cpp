@@ -1439,6 +1439,20 @@ int cgfdm_grid_ranges(int nx, int ny, int nz, const float* coord, const float* m
   return CGFDM_OK;
 }
 
+int cgfdm_grid_ranges_noalloc(int nx, int ny, int nz, const float* coord, const float* medium, double* out8,
+                              void* stream) {
+  if (!coord || !medium || !out8) return fail(CGFDM_ERR_ARG, "null pointer argument");
+  constexpr long long kMaxDim = 1LL << 30, kMaxCells = 1LL << 40;
+  if (nx < 1 || ny < 1 || nz < 1 || nx > kMaxDim || ny > kMaxDim || nz > kMaxDim)
+    return fail(CGFDM_ERR_ARG, "grid dimensions must lie in 1 .. 2^30");
+  const long long cells = (long long)(nx + 2 * cg::H) * (ny + 2 * cg::H);
+  if (cells > kMaxCells / (nz + 2 * cg::H))
+    return fail(CGFDM_ERR_ARG, "padded grid beyond the 2^40-cell maximum of the allocation-free ranges");
+  hipError_t e = cg::grid_ranges_noalloc(nx, ny, nz, coord, medium, out8, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(CGFDM_ERR_HIP, hipGetErrorString(e));
+  return CGFDM_OK;
+}
+
 int cgfdm_destroy(cgfdm_solver* s) {
   if (!s) return CGFDM_OK;
   if (s->impl && s->impl->stream) hipStreamSynchronize(s->impl->stream);

@@ -19,4 +19,12 @@ void launch_hrange(const float* coord, double* part, int nblocks, int nx, int ny
                    hipStream_t st);
 void launch_frange(const float* a, long long n, double* part, int nblocks, hipStream_t st);
 
+// Allocation-free CFL ranges of a device-resident grid: per-block partials in a
+// library-owned 64 KiB device array (RANGE_BLOCKS pairs per reduction), the
+// eight results written to the device array out8. Asynchronous on st.
+constexpr int RANGE_BLOCKS = 1024;
+constexpr int kMaxDevices = 64;
+hipError_t grid_ranges_noalloc(int nx, int ny, int nz, const float* coord, const float* med, double* out8,
+                               hipStream_t st);
+
 }  // namespace cg

@@ -1,6 +1,8 @@
 // Setup kernels: curvilinear metrics, Jacobian, free-surface matrices,
 // coefficient packing and the CFL grid-spacing reduction, all in FP64 on the
 // GPU (the stored coefficients are then rounded to the storage type).
+#include <mutex>
+
 #include "common.h"
 #include "setup.h"
 
@@ -274,6 +276,79 @@ __global__ void k_frange(const float* __restrict__ a, long long n, double* part)
     part[2 * blockIdx.x] = smin[0];
     part[2 * blockIdx.x + 1] = smax[0];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free CFL ranges: the per-block partials go to `scratch`, a
+// file-scope device array (4 reductions x RANGE_BLOCKS (min, max) pairs =
+// 64 KiB) that every GPU the library runs on holds until the process ends.
+// Host code never passes the array's name to a kernel: its device address is
+// looked up once per device with hipGetSymbolAddress and kept per device.
+// Every call on one GPU shares that GPU's array, so calls that overlap on one
+// GPU race; each block overwrites its own partial, so nothing needs clearing.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[8 * RANGE_BLOCKS];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on that device, with the device current).
+hipError_t range_scratch(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return hipSuccess;
+}
+
+// Block c reduces the nb (min, max) partials of reduction c into out8[2c], out8[2c + 1].
+__global__ void k_range_final(const double* __restrict__ part, int nb, double* __restrict__ out8) {
+  __shared__ double smin[256], smax[256];
+  const double* q = part + 2LL * nb * blockIdx.x;
+  double lo = 1e300, hi = -1e300;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    lo = fmin(lo, q[2 * b]);
+    hi = fmax(hi, q[2 * b + 1]);
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out8[2 * blockIdx.x] = smin[0];
+    out8[2 * blockIdx.x + 1] = smax[0];
+  }
+}
+
+hipError_t grid_ranges_noalloc(int nx, int ny, int nz, const float* coord, const float* med, double* out8,
+                               hipStream_t st) {
+  double* part = nullptr;
+  hipError_t e = range_scratch(&part);
+  if (e != hipSuccess) return e;
+  const int NX = nx + 2 * H, NY = ny + 2 * H, NZ = nz + 2 * H;
+  const long long N = (long long)NX * NY * NZ;
+  const int nb = RANGE_BLOCKS;
+  k_hrange<<<nb, 256, 0, st>>>(coord, part, nx, ny, nz, NX, NY, N);
+  for (int c = 0; c < 3; ++c) k_frange<<<nb, 256, 0, st>>>(med + c * N, N, part + 2LL * nb * (c + 1));
+  k_range_final<<<4, 256, 0, st>>>(part, nb, out8);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

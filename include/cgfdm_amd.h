@@ -147,6 +147,20 @@ int cgfdm_timing(cgfdm_solver* s, int enable, double* out);
  * rho_max] of a grid, FP64 on the GPU (calc_CFL.cpp:56-313), for choosing DT. */
 int cgfdm_grid_ranges(int nx, int ny, int nz, const float* coord, const float* medium, double* out8);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): cgfdm_grid_ranges on a grid already in device memory, allocating
+ * nothing. coord and medium are device pointers (float[3][NZ][NY][NX] each, the
+ * layout of cgfdm_grid_ranges), out8 is a device pointer to 8 doubles that receives
+ * [Hmin, Hmax, Vs_min, Vs_max, Vp_min, Vp_max, rho_min, rho_max]; the work is
+ * enqueued on `stream` (hipStream_t as void*, NULL = default stream) and the call
+ * returns without waiting. The per-block partials live in a 64 KiB device array of
+ * the library that each GPU holds until the process ends and that every call on
+ * that GPU shares: calls that overlap on one GPU race. Limits: nx, ny, nz in
+ * 1 .. 2^30 and at most 2^40 padded cells, else CGFDM_ERR_ARG. */
+int cgfdm_grid_ranges_noalloc(int nx, int ny, int nz, const float* coord, const float* medium, double* out8,
+                              void* stream);
+
 /* Device bytes held by the solver. */
 long long cgfdm_device_bytes(cgfdm_solver* s);
 
