@@ -41,8 +41,9 @@ RUNNER = os.path.join(PKG, "cgfdm_run")
 
 def build_runner(verbose=True):
     """The params.json driver (C++ host on the C ABI), linked against the library
-    next to it; the CPU-only host sources are also used by the host tests."""
-    srcs = [os.path.join(HOST, f) for f in HOST_SOURCES + ["cgfdm_run.cpp"]]
+    next to it; the host sources it relies on are inside the library (and are
+    also built on their own by the host tests)."""
+    srcs = [os.path.join(HOST, "cgfdm_run.cpp")]
     tmp = RUNNER + ".tmp"
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, "-I", HOST, "-o", tmp] + srcs + \
           ["-L", PKG, "-lcgfdm_amd", "-Wl,-rpath,$ORIGIN"]
@@ -55,12 +56,29 @@ def build_runner(verbose=True):
     return RUNNER
 
 
+# host sources linked into the library: the params.json session entry points
+# (cgfdm_create_from_params, cgfdm_write_outputs) and the readers, builders and
+# writers they use
+LIB_HOST_SOURCES = HOST_SOURCES + ["session.cpp"]
+
+
+def _compile_host(src):
+    out = os.path.join(OBJ, "host_" + os.path.splitext(src)[0] + ".o")
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-I", INCLUDE, "-I", HOST, "-c", os.path.join(HOST, src),
+           "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{r.stderr}")
+    return out, r.stderr
+
+
 def build(verbose=True, jobs=None):
     os.makedirs(OBJ, exist_ok=True)
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
         results = list(ex.map(_compile, SOURCES))
-    objs = [o for o, _ in results]
+        host_results = list(ex.map(_compile_host, LIB_HOST_SOURCES))
+    objs = [o for o, _ in results] + [o for o, _ in host_results]
     tmp = LIB + ".tmp"
     # -z defs: an unresolved symbol fails the link instead of the first load
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp] + objs

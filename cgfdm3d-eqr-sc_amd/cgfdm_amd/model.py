@@ -390,6 +390,53 @@ def config3(nt=4000, nx=512, ny=512, nz=256, dh=100.0, seed=1002, nhills=8, hmin
     return m
 
 
+def config4(nt=2000, nx=1024, ny=1024, nz=320, dh=50.0, npml=20, seed=1004, hmin_fn=None):
+    """Mixed-precision large domain: 1024 x 1024 x 320 at h = 50 m (16 km deep,
+    335.5 M cells), 8 seeded Gaussian hills, the 4-layer medium of config2 with a
+    5% Gaussian random perturbation (1 km correlation), 20-layer PML; a finite
+    fault of 40 x 20 subfaults on a 40 x 20 km plane (strike 30, dip 40, rake 90,
+    top edge at 1 km depth, hypocentre at its centre) with seeded von Karman slip
+    (a_s 10 km, a_d 5 km, H 0.75, mean 2 m), rupture at 0.8 Vs and a 1 s
+    triangular slip rate; 400 surface stations on a 20 x 20 grid plus 16 buried
+    stations on a 4 x 4 grid of 1 km spacing, 1 km above the hypocentre (they
+    see the first arrivals within a few hundred steps). Meant to be stepped with
+    16-bit coefficient storage; the FP64 run of the same kernels is its
+    reference."""
+    depth = nz * dh
+    terrain = gaussian_hills(nx, ny, dh, npml, 8, seed)
+    coord = terrain_coord(nx, ny, nz, dh, terrain, depth)
+    vp = np.array([3000.0, 4500.0, 6000.0, 7000.0])
+    med = layered_medium(coord, terrain, [0.0, 2000.0, 8000.0, 12000.0], vp / 1.73, vp,
+                         [2300.0, 2500.0, 2700.0, 3000.0])
+    delta = gaussian_random_field(med.shape[1:], dh, 1000.0, 0.05, seed + 7)
+    med[0] *= 1.0 + delta
+    med[1] *= 1.0 + delta
+    del delta
+    hmin = (hmin_fn or (lambda c, md: hmin_point_plane(c)))(coord, med)
+    dt = 0.5 * hmin / float(med[1].max())
+    m = Model(nx, ny, nz, dh, dt, nt, npml, coord, med, True, name="config4", terrain=terrain)
+    nl, nw, length, width, dip = 40, 20, 40000.0, 20000.0, 40.0
+    slip = von_karman_slip(nl, nw, length / nl, width / nw, 10000.0, 5000.0, 0.75, 2.0, seed)
+    xc, yc = (nx // 2) * dh, (ny // 2) * dh
+    # rupture speed from the shear speed at the hypocentre's depth range
+    hyp_depth = 1000.0 + 0.5 * width * math.sin(math.radians(dip))
+    i0, j0, k0 = nearest_cell(m, xc, yc, -hyp_depth)
+    vs_src = float(np.median(med[0, max(k0 - 20, 0):k0 + 21, max(j0 - 20, 0):j0 + 21, max(i0 - 20, 0):i0 + 21]))
+    m.src_mt = finite_fault(m, xc, yc, 1000.0, 30.0, dip, 90.0, length, width, nl, nw, slip, 0.8 * vs_src, 1.0)
+    # hypocentre: centre of the plane (finite_fault's default hypo = (0.5, 0.5))
+    s_, d_ = math.radians(30.0), math.radians(dip)
+    u_d = np.array([math.cos(s_) * math.cos(d_), -math.sin(s_) * math.cos(d_), -math.sin(d_)])
+    hyp = np.array([xc, yc, -1000.0]) + 0.5 * width * u_d
+    buried = []
+    for b in range(4):
+        for a in range(4):
+            x = hyp[0] + (a - 1.5) * 1000.0
+            y = hyp[1] + (b - 1.5) * 1000.0
+            buried.append(nearest_cell(m, x, y, hyp[2] + 1000.0))
+    m.stations = np.concatenate([surface_station_grid(nx, ny, nz, npml, 20), np.array(buried, np.int32)])
+    return m
+
+
 def config5(nt=2000, nx=256, ny=256, nz=128, dh=100.0, npml=20, seed=1005, hmin_fn=None):
     """Adjoint workload: 256 x 256 x 128 at h = 100 m, 4 seeded Gaussian hills,
     4-layer medium with a 5% Gaussian random perturbation (1 km correlation),

@@ -42,7 +42,8 @@ EXPORTS = ["cgfdm_create", "cgfdm_destroy", "cgfdm_set_source", "cgfdm_set_stati
            "cgfdm_get_pgv", "cgfdm_get_surface_displacement", "cgfdm_get_field", "cgfdm_set_field",
            "cgfdm_get_setup", "cgfdm_get_ranges", "cgfdm_timing", "cgfdm_device_bytes",
            "cgfdm_last_error", "cgfdm_grid_ranges", "cgfdm_forward_record", "cgfdm_adjoint", "cgfdm_get_slice",
-           "cgfdm_grid_ranges_noalloc"]
+           "cgfdm_grid_ranges_noalloc", "cgfdm_kernel_name", "cgfdm_create_from_params", "cgfdm_write_outputs",
+           "cgfdm_run_info"]
 
 
 def lib():
@@ -76,6 +77,8 @@ def lib():
     L.cgfdm_last_error.restype = ctypes.c_char_p
     L.cgfdm_grid_ranges.argtypes = [i, i, i, vp, vp, vp]
     L.cgfdm_grid_ranges_noalloc.argtypes = [i, i, i, vp, vp, vp, vp]
+    L.cgfdm_kernel_name.restype = ctypes.c_char_p
+    L.cgfdm_kernel_name.argtypes = [vp, i]
     L.cgfdm_forward_record.argtypes = [vp, i, i, vp]
     L.cgfdm_adjoint.argtypes = [vp, vp, vp, vp, vp, vp]
     _lib = L
@@ -255,6 +258,11 @@ class Solver:
         _check(self.L.cgfdm_adjoint(self.h, _ptr(src), _ptr(gmu), _ptr(glam), _ptr(gb), _ptr(grate)))
         return dict(gmu=gmu.reshape(self.shape), glam=glam.reshape(self.shape), gb=gb.reshape(self.shape),
                     grate=grate[:npts])
+
+    def kernel_name(self, which=0):
+        """Name of the kernel the solver launches: 0 = the interior (CGFDM) or
+        face-flux (ME-AWENO) kernel, 1 = the surface or update kernel."""
+        return self.L.cgfdm_kernel_name(self.h, int(which)).decode()
 
     def device_bytes(self):
         return int(self.L.cgfdm_device_bytes(self.h))

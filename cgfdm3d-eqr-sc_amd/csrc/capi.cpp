@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <string>
@@ -27,6 +28,16 @@ int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+
+}  // namespace
+
+// hooks of the params.json session code (host/session.cpp), linked into the library
+namespace cgfdm_detail {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+void release_session(cgfdm_solver* s);
+}  // namespace cgfdm_detail
+
+namespace {
 
 // Code paths that can be switched off as a whole: stepping returns
 // CGFDM_ERR_DISABLED without launching a kernel. Both are on; the switch stays
@@ -80,6 +91,7 @@ struct SolverBase {
   virtual int get_setup(int what, double* out) = 0;
   // one component of the current state on an interior plane, physical units
   virtual int get_slice(int comp, int axis, int index, float* out) = 0;
+  virtual const char* kernel_name(int which) const = 0;
   virtual int forward_record(int, int, double*) { return fail(CGFDM_ERR_ARG, "the adjoint is available for the CGFDM scheme"); }
   virtual int adjoint(const double*, double*, double*, double*, double*) {
     return fail(CGFDM_ERR_ARG, "the adjoint is available for the CGFDM scheme");
@@ -285,6 +297,28 @@ float to_float<__half>(__half v) { return __half2float(v); }
 template <>
 float to_float<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
 
+// host decoding of the 16-bit codes (IEEE binary16 / bfloat16 bit patterns)
+template <class TM>
+float bits16_to_float(unsigned short b);
+template <>
+float bits16_to_float<__half>(unsigned short b) {
+  const unsigned man = b & 0x3ffu, ex = (b >> 10) & 0x1fu;
+  float v;
+  if (ex == 0) v = std::ldexp((float)man, -24);
+  else if (ex == 31) v = man ? std::numeric_limits<float>::quiet_NaN() : std::numeric_limits<float>::infinity();
+  else v = std::ldexp((float)(man | 0x400u), (int)ex - 25);
+  return (b & 0x8000u) ? -v : v;
+}
+template <>
+float bits16_to_float<__hip_bfloat16>(unsigned short b) {
+  const unsigned u = (unsigned)b << 16;
+  float v;
+  std::memcpy(&v, &u, sizeof v);
+  return v;
+}
+template <class TM>
+float bits16_to_float(unsigned short) { return 0.f; }
+
 template <class R, class TM>
 struct Solver : SolverBase {
   cgfdm_desc d{};
@@ -298,6 +332,13 @@ struct Solver : SolverBase {
   double Z = 1.0;  // stress carried as T / Z
   R *A = nullptr, *B = nullptr, *C = nullptr, *T = nullptr;
   TM* coef = nullptr;
+  // 16-bit storage: coef points at six word planes of pair codes, aff at the
+  // affine table [12][NZ][2] (common.h, Bits16)
+  static constexpr bool kPacked16 = sizeof(TM) == 2;
+  float* aff = nullptr;
+  // 16-bit storage: mu / Z in FP32 (tight layout), read only by set_source so
+  // that the moment-rate scaling mu / (J DH^3) does not carry the 16-bit error
+  float* mu_src = nullptr;
   R* jac = nullptr;
   R *rdzdx = nullptr, *rdzdy = nullptr;
   R* pml_par = nullptr;  // alpha, beta, d for x (NX each), then y, then z
@@ -331,15 +372,20 @@ struct Solver : SolverBase {
   // adjoint then reads them instead of recomputing three stages per step
   R* adj_stages = nullptr;
 
-  // interior block order: regions of 64 x (4 rty) columns per XCD; the
-  // environment variable CGFDM_INTERIOR_RTY (1..64) overrides it for tuning
-  int rty = 4;
-  int rtx = 1;
+  // interior block order: regions of (64 rtx) x (4 rty) columns per XCD; the
+  // environment variables CGFDM_INTERIOR_RTX / _RTY (1..64) override it
+  // (rtx 8, rty 2: 512 x 8-column regions, measured fastest on the 512 x 512 x
+  // 256 workload: 4.43 ms per launch against 5.08 ms for 64 x 16 regions)
+  int rty = 2;
+  int rtx = 8;
   // interior kernel choice (FP32 wavefield): 0 thread-per-cell, -1 x-paired
   // (FP32 coefficients, even NX), 1 plane tile, > 1 z-marching with chunks of kc
   // planes. CGFDM_INTERIOR_KC (-1..4096) overrides it.
   int kc = 0;
   int zty = 8;  // z-marching tile height (CGFDM_INTERIOR_ZTY: 4 or 8)
+  // register z-marching kernel: planes per chunk (0 = off); CGFDM_INTERIOR_ZREG
+  // (0..4096) overrides it
+  int zreg = 0;
 
   int init(const cgfdm_desc& dd, const float* coord, const float* medium) override {
     d = dd;
@@ -356,6 +402,10 @@ struct Solver : SolverBase {
       if (v >= -1 && v <= 4096) kc = v;
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
+    if (const char* e = std::getenv("CGFDM_INTERIOR_ZREG")) {
+      const int v = std::atoi(e);
+      if (v >= 0 && v <= 4096) zreg = v;
+    }
     if (const char* e = std::getenv("CGFDM_ROW_ALIGN")) galign = std::atoi(e) != 0;
     // the x-paired kernel pairs cells (i, i + 1) with even i on 8-byte
     // boundaries, which the tight layout gives and the row-aligned one does not
@@ -394,10 +444,18 @@ struct Solver : SolverBase {
 
     // --- packed coefficients (tight), then the row-aligned copies
     TM* coef_t = nullptr;
+    unsigned* words_t = nullptr;
     R *jac_t = nullptr, *rdzdx_t = nullptr, *rdzdy_t = nullptr;
-    if ((rc = dalloc(&coef_t, (long long)cg::M_NCOEF * N))) return rc;
     if ((rc = dalloc(&jac_t, N))) return rc;
-    cg::launch_pack<R, TM>(met, dmed, coef_t, jac_t, N, Z, stream);
+    if constexpr (kPacked16) {
+      if ((rc = dalloc(&aff, 2LL * cg::M_NCOEF * NZ))) return rc;
+      if ((rc = dalloc(&words_t, (long long)(cg::M_NCOEF / 2) * N))) return rc;
+      if ((rc = dalloc(&mu_src, N))) return rc;
+      cg::launch_pack16<TM>(met, dmed, aff, words_t, (float*)jac_t, mu_src, NX, NY, NZ, Z, stream);
+    } else {
+      if ((rc = dalloc(&coef_t, (long long)cg::M_NCOEF * N))) return rc;
+      cg::launch_pack<R, TM>(met, dmed, coef_t, jac_t, N, Z, stream);
+    }
     if ((rc = dalloc(&rdzdx_t, 9 * S))) return rc;
     if ((rc = dalloc(&rdzdy_t, 9 * S))) return rc;
     if (d.free_surface) cg::launch_rdz<R>(met, dmed, rdzdx_t, rdzdy_t, d.nx, d.ny, d.nz, NX, NY, N, stream);
@@ -409,20 +467,32 @@ struct Solver : SolverBase {
     NX = galign ? (tNX + 31) / 32 * 32 : tNX;
     S = (long long)NX * NY;
     N = S * NZ;
+    // the stage kernels address grid arrays with 32-bit byte offsets per plane
+    if ((double)N * (double)sizeof(R) >= 4294967296.0)
+      return fail(CGFDM_ERR_ARG, "grid too large: one field plane array must stay below 4 GiB (" +
+                                     std::to_string(N) + " padded cells)");
     pNX = NX;
     pNY = NY;
     pNZ = NZ;
     pXe = tNX;
-    if ((rc = galloc(&coef, (long long)cg::M_NCOEF * N))) return rc;
+    if constexpr (kPacked16) {
+      unsigned* words = nullptr;
+      if ((rc = galloc(&words, (long long)(cg::M_NCOEF / 2) * N))) return rc;
+      if ((rc = repack(words, words_t, (long long)(cg::M_NCOEF / 2) * NY * NZ, true))) return rc;
+      coef = reinterpret_cast<TM*>(words);
+    } else {
+      if ((rc = galloc(&coef, (long long)cg::M_NCOEF * N))) return rc;
+      if ((rc = repack(coef, coef_t, (long long)cg::M_NCOEF * NY * NZ, true))) return rc;
+    }
     if ((rc = galloc(&jac, N))) return rc;
     if ((rc = galloc(&rdzdx, 9 * S))) return rc;
     if ((rc = galloc(&rdzdy, 9 * S))) return rc;
-    if ((rc = repack(coef, coef_t, (long long)cg::M_NCOEF * NY * NZ, true))) return rc;
     if ((rc = repack(jac, jac_t, (long long)NY * NZ, true))) return rc;
     if ((rc = repack(rdzdx, rdzdx_t, 9LL * NY, true))) return rc;
     if ((rc = repack(rdzdy, rdzdy_t, 9LL * NY, true))) return rc;
     HIPCHK(hipStreamSynchronize(stream));
-    dfree(coef_t);
+    if (coef_t) dfree(coef_t);
+    if (words_t) dfree(words_t);
     dfree(jac_t);
     dfree(rdzdx_t);
     dfree(rdzdy_t);
@@ -501,6 +571,26 @@ struct Solver : SolverBase {
     return CGFDM_OK;
   }
 
+  // coefficient c of device cell `cell` as the kernels see it (16-bit storage
+  // decoded with its layer's affine pair)
+  int coef_host(int c, long long cell, double& out) {
+    if constexpr (kPacked16) {
+      unsigned w = 0;
+      float A[2] = {0.f, 0.f};
+      const long long k = cell / S;
+      HIPCHK(hipMemcpy(&w, reinterpret_cast<const unsigned*>(coef) + (long long)(c >> 1) * N + cell, sizeof(w),
+                       hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(A, aff + 2 * ((long long)c * NZ + k), sizeof(A), hipMemcpyDeviceToHost));
+      const unsigned short b = (c & 1) ? (unsigned short)(w >> 16) : (unsigned short)(w & 0xffffu);
+      out = (double)(A[0] + A[1] * bits16_to_float<TM>(b));
+    } else {
+      TM v;
+      HIPCHK(hipMemcpy(&v, coef + (long long)c * N + cell, sizeof(TM), hipMemcpyDeviceToHost));
+      out = (double)v;
+    }
+    return CGFDM_OK;
+  }
+
   int set_source(int np, int nt, double dt, const long long* cell, const float* rate) override {
     if (np < 0 || (np > 0 && (nt < 2 || dt <= 0 || !cell || !rate))) return fail(CGFDM_ERR_SOURCE, "bad source table");
     std::map<long long, std::vector<double>> acc;
@@ -533,11 +623,17 @@ struct Solver : SolverBase {
       merged[c] = Jp;
       hc.push_back(c);
       R jv;
-      TM mv;
       HIPCHK(hipMemcpy(&jv, jac + c, sizeof(R), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(&mv, coef + (long long)cg::M_MU * N + c, sizeof(TM), hipMemcpyDeviceToHost));
-      double mu = (double)to_float(mv);
-      if (sizeof(TM) == 8) mu = *(double*)(void*)&mv;
+      double mu = 0.0;
+      if constexpr (kPacked16) {
+        // mu_src keeps the tight layout of the setup stage
+        const long long ct = c % NX + (c / NX % NY) * (long long)tNX + c / S * tS;
+        float mv = 0.f;
+        HIPCHK(hipMemcpy(&mv, mu_src + ct, sizeof(float), hipMemcpyDeviceToHost));
+        mu = mv;
+      } else if (int rc2 = coef_host(cg::M_MU, c, mu)) {
+        return rc2;
+      }
       double J = (double)jv;
       double sc = mu / (J * d.dh * d.dh * d.dh);  // mu/(J DH^3), stress in units of Z
       src_muv[Jp] = mu;
@@ -582,6 +678,15 @@ struct Solver : SolverBase {
 
   int step(int n) override { return step_impl(n, true); }
 
+  const char* kernel_name(int which) const override {
+    if (which == 1) return "cg_surface";
+    if (which != 0) return nullptr;
+    if (std::is_same<R, float>::value && std::is_same<TM, float>::value && kc < 0 && !galign) return "cg_pair";
+    if (std::is_same<R, float>::value && kc == 1) return "cg_plane";
+    if (std::is_same<R, float>::value && kc > 1) return "cg_zcol";
+    return zreg > 0 ? "cg_zreg" : "cg_cell";
+  }
+
   // observe = false: no station record, PGV or displacement update (used to
   // recompute states for the adjoint)
   int step_impl(int n, bool observe) {
@@ -615,15 +720,20 @@ struct Solver : SolverBase {
         a.NY = NY;
         a.N = N;
         a.S = S;
+        a.rowb = (unsigned)(NX * sizeof(R));
+        a.planeb = (unsigned)(S * sizeof(R));
         a.kbeg = cg::H;
         a.kend = kend;
         a.stage = irk;
         a.rty = rty;
         a.rtx = rtx;
+        a.zreg = zreg;
         a.kc = kc;
         a.zty = zty;
         a.DT = R(d.dt);
         a.rDH = R(1.0 / d.dh);
+        a.aff = aff;
+        a.nzp = NZ;
         a.win = irk == 0 ? A : irk == 1 ? B : irk == 2 ? C : B;
         a.wout = irk == 0 ? B : irk == 1 ? C : irk == 2 ? B : A;
         a.wm = A;
@@ -716,6 +826,21 @@ struct Solver : SolverBase {
 
   int get_setup(int what, double* out) override {
     HIPCHK(hipStreamSynchronize(stream));
+    if (what >= 0 && what <= 12 && what != 9 && kPacked16) {
+      const int plane = what < 9 ? what : what - 1;
+      std::vector<unsigned> h(tN);
+      std::vector<float> ha(2 * (size_t)cg::M_NCOEF * NZ);
+      if (int rc = to_host(reinterpret_cast<const unsigned*>(coef) + (size_t)(plane >> 1) * N, h.data(), 1)) return rc;
+      HIPCHK(hipMemcpy(ha.data(), aff, sizeof(float) * ha.size(), hipMemcpyDeviceToHost));
+      const double sc = what == 10 || what == 11 ? Z : what == 12 ? 1.0 / Z : 1.0;
+      for (long long p = 0; p < tN; ++p) {
+        const int k = (int)(p / tS);
+        const unsigned short b = (plane & 1) ? (unsigned short)(h[p] >> 16) : (unsigned short)(h[p] & 0xffffu);
+        const float* A = ha.data() + 2 * ((size_t)plane * NZ + k);
+        out[p] = (double)(A[0] + A[1] * bits16_to_float<TM>(b)) * sc;
+      }
+      return CGFDM_OK;
+    }
     if (what >= 0 && what <= 12 && what != 9) {
       int plane = what < 9 ? what : what - 1;  // 10,11,12 -> M_MU.. (9,10,11)
       std::vector<TM> h(tN);
@@ -805,8 +930,9 @@ struct Solver : SolverBase {
     for (int irk = 0; irk < 4; ++irk) {
       cg::StageArgs<R> a{};
       a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.NX = NX; a.NY = NY; a.N = N; a.S = S;
-      a.kbeg = cg::H; a.kend = kend; a.stage = irk; a.rty = rty; a.rtx = rtx; a.kc = kc; a.zty = zty;
-      a.DT = R(d.dt); a.rDH = R(1.0 / d.dh);
+      a.rowb = (unsigned)(NX * sizeof(R)); a.planeb = (unsigned)(S * sizeof(R));
+      a.kbeg = cg::H; a.kend = kend; a.stage = irk; a.rty = rty; a.rtx = rtx; a.zreg = zreg; a.kc = kc; a.zty = zty;
+      a.DT = R(d.dt); a.rDH = R(1.0 / d.dh); a.aff = aff; a.nzp = NZ;
       a.win = ins[irk]; a.wout = outs[irk]; a.wm = A; a.wt = T; a.pml = pml;
       LAUNCHCHK((cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream)), "interior stage");
       if (d.free_surface) {
@@ -841,8 +967,9 @@ struct Solver : SolverBase {
     for (int irk = 0; irk < 3; ++irk) {
       cg::StageArgs<R> a{};
       a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.NX = NX; a.NY = NY; a.N = N; a.S = S;
-      a.kbeg = cg::H; a.kend = kend; a.stage = irk; a.rty = rty; a.rtx = rtx; a.kc = kc; a.zty = zty;
-      a.DT = R(d.dt); a.rDH = R(1.0 / d.dh);
+      a.rowb = (unsigned)(NX * sizeof(R)); a.planeb = (unsigned)(S * sizeof(R));
+      a.kbeg = cg::H; a.kend = kend; a.stage = irk; a.rty = rty; a.rtx = rtx; a.zreg = zreg; a.kc = kc; a.zty = zty;
+      a.DT = R(d.dt); a.rDH = R(1.0 / d.dh); a.aff = aff; a.nzp = NZ;
       a.win = ins[irk]; a.wout = outs[irk]; a.wm = A; a.wt = T; a.pml = pml;
       LAUNCHCHK((cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream)), "interior stage");
       if (d.free_surface) {
@@ -1066,8 +1193,8 @@ struct AwenoSolver : SolverBase {
   R* src_rate = nullptr;
   std::vector<double> jac_host;  // FP64 J for the source scaling
 
-  // interior block order: regions of 64 x (4 rty) columns per XCD; the
-  // environment variable CGFDM_INTERIOR_RTY (1..64) overrides it for tuning
+  // interior block order: regions of (64 rtx) x (4 rty) columns per XCD; the
+  // environment variables CGFDM_INTERIOR_RTX / _RTY (1..64) override it
   int rty = 4;
   int rtx = 1;
   // interior kernel choice (FP32 wavefield): 0 thread-per-cell, -1 x-paired
@@ -1202,6 +1329,10 @@ struct AwenoSolver : SolverBase {
     if ((rc = dalloc(&st_rec, (long long)n * d.nt * 9))) return rc;
     HIPCHK(hipMemcpy(st_cell, hc.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
     return CGFDM_OK;
+  }
+
+  const char* kernel_name(int which) const override {
+    return which == 0 ? "aw_faces" : which == 1 ? "aw_update" : nullptr;
   }
 
   int step(int n) override {
@@ -1453,8 +1584,14 @@ int cgfdm_grid_ranges_noalloc(int nx, int ny, int nz, const float* coord, const 
   return CGFDM_OK;
 }
 
+const char* cgfdm_kernel_name(cgfdm_solver* s, int which) {
+  if (!s || !s->impl) return nullptr;
+  return s->impl->kernel_name(which);
+}
+
 int cgfdm_destroy(cgfdm_solver* s) {
   if (!s) return CGFDM_OK;
+  cgfdm_detail::release_session(s);
   if (s->impl && s->impl->stream) hipStreamSynchronize(s->impl->stream);
   delete s;
   return CGFDM_OK;

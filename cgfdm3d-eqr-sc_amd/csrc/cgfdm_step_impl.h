@@ -110,20 +110,78 @@ template <class T>
 __device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
 template <class T>
 __device__ __forceinline__ void st_stream(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
+// Access at a byte offset from a uniform base pointer: the compiler emits the
+// global load/store form with a 64-bit SGPR base and a 32-bit VGPR offset, so
+// one offset register serves every plane of every grid array (field, RK,
+// coefficient). Every grid plane array spans less than 4 GiB (checked on the
+// host before a launch), so byte offsets of cells fit 32 bits.
 template <class T>
-__device__ __forceinline__ float ld_coef(const T* p) { return Store<T>::ld(p); }
-template <>
-__device__ __forceinline__ float ld_coef<float>(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ const T* at(const T* base, unsigned boff) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <class T>
+__device__ __forceinline__ T* at(T* base, unsigned boff) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff);
+}
+
+// Byte offset of padded cell (i, j, k) in a grid array of R elements (16-bit
+// coefficient words are 4 bytes, as R = float in that storage).
+template <class R>
+__device__ __forceinline__ unsigned cell_off(const StageArgs<R>& a, int i, int j, int k) {
+  return (unsigned)i * (unsigned)sizeof(R) + (unsigned)j * a.rowb + (unsigned)k * a.planeb;
+}
+
+// 5-point MacCormack difference (without 1/DH) of one plane array around byte
+// offset o with a byte stride st between stencil points.
+template <int FB, class R>
+__device__ __forceinline__ R mac_off(const R* base, unsigned o, unsigned st) {
+  const unsigned d = FB > 0 ? st : 0u - st;
+  return R(FB) * (R(AF_1) * *at(base, o - d) + R(AF0) * *at(base, o) + R(AF1) * *at(base, o + d) +
+                  R(AF2) * *at(base, o + 2 * d) + R(AF3) * *at(base, o + 3 * d));
+}
+
+// Coefficient c of the cell at byte offset `off` in layer k. Native storage:
+// plane c (with the non-temporal hint when NT is set). 16-bit storage: the half
+// of word plane c / 2 that holds plane c, decoded with the layer's affine pair.
+template <class R, class TM, bool NT>
+__device__ __forceinline__ R coef_off(const TM* __restrict__ coef, const StageArgs<R>& a, int c, unsigned off, int k) {
+  if constexpr (sizeof(TM) == 2) {
+    static_assert(sizeof(R) == 4, "16-bit coefficient words are addressed like an FP32 grid array");
+    const unsigned* w = at(reinterpret_cast<const unsigned*>(coef) + (long long)(c >> 1) * a.N, off);
+    const unsigned word = NT ? __builtin_nontemporal_load(w) : *w;
+    const unsigned short b = (c & 1) ? (unsigned short)(word >> 16) : (unsigned short)(word & 0xffffu);
+    const float* A = a.aff + 2 * ((long long)c * a.nzp + k);
+    return R(A[0] + A[1] * Bits16<TM>::to_float(b));
+  } else {
+    static_assert(sizeof(TM) == sizeof(R), "native coefficients share the grid byte offsets");
+    const TM* q = at(coef + (long long)c * a.N, off);
+    if constexpr (NT) return R(__builtin_nontemporal_load(q));
+    else return R(*q);
+  }
+}
+
+// Coefficient c of cell q (linear index, in layer k), for the surface kernel.
+template <class R, class TM, bool NT>
+__device__ __forceinline__ R coef_at(const TM* __restrict__ coef, const StageArgs<R>& a, int c, long long q, int k) {
+  if constexpr (sizeof(TM) == 2) {
+    const unsigned* w = reinterpret_cast<const unsigned*>(coef) + (long long)(c >> 1) * a.N + q;
+    const unsigned word = NT ? __builtin_nontemporal_load(w) : *w;
+    const unsigned short b = (c & 1) ? (unsigned short)(word >> 16) : (unsigned short)(word & 0xffffu);
+    const float* A = a.aff + 2 * ((long long)c * a.nzp + k);
+    return R(A[0] + A[1] * Bits16<TM>::to_float(b));
+  } else if constexpr (NT && sizeof(TM) == 4) {
+    return R(__builtin_nontemporal_load(coef + (long long)c * a.N + q));
+  } else {
+    return R(coef[(long long)c * a.N + q]);
+  }
+}
 
 template <class R, class TM, int FB1, int FB2, int FB3, int STAGE, class XD, class YD, class ZD, class WC>
 __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __restrict__ coef, int i, int j, int k,
-                                          XD xd, YD yd, ZD zd, WC wc) {
-  const long long N = a.N, S = a.S;
-  const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
-  auto cf = [&](int c) -> R {
-    if constexpr (sizeof(TM) == 4) return R(ld_coef(coef + (long long)c * N + p));
-    else return R(Store<TM>::ld(coef + (long long)c * N + p));
-  };
+                                          unsigned off, XD xd, YD yd, ZD zd, WC wc) {
+  const long long N = a.N;
+  auto cf = [&](int c) -> R { return coef_off<R, TM, true>(coef, a, c, off, k); };
   const R mu = cf(M_MU), lam = cf(M_LAM), bu = cf(M_B);
   const PmlArgs<R>& P = a.pml;
   R h[9], D[9], Q[9];
@@ -189,19 +247,20 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
   // RK4 stage update of W (wave_rk.cpp:18-104)
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
-    const long long q = (long long)v * N + p;
+    R* wt = at(a.wt + (long long)v * N, off);
+    R* wo = at(a.wout + (long long)v * N, off);
     if constexpr (STAGE == 0) {
       const R m = wc(v);
-      st_stream(a.wt + q, R(m + R(RK_B1) * h[v]));
-      st_stream(a.wout + q, R(m + R(RK_A2) * h[v]));
+      st_stream(wt, R(m + R(RK_B1) * h[v]));
+      st_stream(wo, R(m + R(RK_A2) * h[v]));
     } else if constexpr (STAGE == 1) {
-      st_stream(a.wt + q, R(ld_stream(a.wt + q) + R(RK_B2) * h[v]));
-      st_stream(a.wout + q, R(ld_stream(a.wm + q) + R(RK_A3) * h[v]));
+      st_stream(wt, R(ld_stream((const R*)wt) + R(RK_B2) * h[v]));
+      st_stream(wo, R(ld_stream(at(a.wm + (long long)v * N, off)) + R(RK_A3) * h[v]));
     } else if constexpr (STAGE == 2) {
-      st_stream(a.wt + q, R(ld_stream(a.wt + q) + R(RK_B3) * h[v]));
-      st_stream(a.wout + q, R(ld_stream(a.wm + q) + h[v]));
+      st_stream(wt, R(ld_stream((const R*)wt) + R(RK_B3) * h[v]));
+      st_stream(wo, R(ld_stream(at(a.wm + (long long)v * N, off)) + h[v]));
     } else {
-      st_stream(a.wout + q, R(ld_stream(a.wt + q) + R(RK_B4) * h[v]));
+      st_stream(wo, R(ld_stream((const R*)wt) + R(RK_B4) * h[v]));
     }
   }
 }
@@ -254,15 +313,99 @@ __global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restr
   const int j = H + t.j0 + (threadIdx.x >> 6);
   const int k = a.kbeg + t.k;
   if (i >= H + a.nx || j >= H + a.ny) return;
-  const long long N = a.N, S = a.S;
-  const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
+  const long long N = a.N;
+  const unsigned off = cell_off(a, i, j, k);
   const R* W = a.win;
-  const long long NXl = a.NX;
   cell_body<R, TM, FB1, FB2, FB3, STAGE>(
-      a, coef, i, j, k, [&](int v) -> R { return mac<FB1>(W + (long long)v * N + p, 1); },
-      [&](int v) -> R { return mac<FB2>(W + (long long)v * N + p, NXl); },
-      [&](int v) -> R { return mac<FB3>(W + (long long)v * N + p, S); },
-      [&](int v) -> R { return W[(long long)v * N + p]; });
+      a, coef, i, j, k, off,
+      [&](int v) -> R { return mac_off<FB1>(W + (long long)v * N, off, (unsigned)sizeof(R)); },
+      [&](int v) -> R { return mac_off<FB2>(W + (long long)v * N, off, a.rowb); },
+      [&](int v) -> R { return mac_off<FB3>(W + (long long)v * N, off, a.planeb); },
+      [&](int v) -> R { return *at(W + (long long)v * N, off); });
+}
+
+// ---------------------------------------------------------------------------
+// register z-marching interior kernel: a block of 64 x 4 threads owns a 64 x 4
+// column tile and marches up through a chunk of zc planes. The z stencil of
+// every cell comes from a five-plane register window of the thread's own
+// column (one new plane of the nine fields per cell, loaded one plane ahead),
+// so the z neighbours are read from HBM once instead of once per stencil
+// point; the x and y stencils are read from global memory and hit L1/L2.
+// No LDS and no barriers. Blocks are dealt to the XCDs in regions of rtx x rty
+// tiles; all tiles of a region march the same chunk together, so the y and x
+// halo rows of a tile are the rows its neighbours are loading at that moment.
+// ---------------------------------------------------------------------------
+struct ZregTile {
+  int i0, j0, k0, k1;
+};
+__device__ __forceinline__ bool zreg_tile(int b, int nx, int ny, int kbeg, int kend, int zc, int rtx, int rty,
+                                          ZregTile& t) {
+  const int ntx = (nx + 63) / 64, nty = (ny + 3) / 4, nch = (kend - kbeg + zc - 1) / zc;
+  const int nrx = (ntx + rtx - 1) / rtx, nry = (nty + rty - 1) / rty;
+  const int per_chunk = rtx * rty, per_region = per_chunk * nch;
+  const int xcd = b & 7, seq = b >> 3;
+  const int slot = seq / per_region, within = seq - slot * per_region;
+  const int r = slot * 8 + xcd;
+  const int ch = within / per_chunk, q = within - ch * per_chunk;
+  const int ty = q / rtx, tx = q - ty * rtx;
+  const int rx = r % nrx, ry = r / nrx;
+  const int txg = rx * rtx + tx, tyg = ry * rty + ty;
+  t.i0 = txg * 64;
+  t.j0 = tyg * 4;
+  t.k0 = kbeg + ch * zc;
+  t.k1 = min(t.k0 + zc, kend);
+  return r < nrx * nry && txg < ntx && tyg < nty;
+}
+
+inline long long zreg_blocks(int nx, int ny, int nk, int zc, int rtx, int rty) {
+  const long long ntx = (nx + 63) / 64, nty = (ny + 3) / 4, nch = (nk + zc - 1) / zc;
+  const long long nregions = ((ntx + rtx - 1) / rtx) * ((nty + rty - 1) / rty);
+  return 8 * ((nregions + 7) / 8) * (long long)rtx * rty * nch;
+}
+
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void cg_zreg(StageArgs<R> a, const TM* __restrict__ coef) {
+  ZregTile t;
+  if (!zreg_tile(blockIdx.x, a.nx, a.ny, a.kbeg, a.kend, a.zreg, a.rtx, a.rty, t)) return;
+  const int i = H + t.i0 + (threadIdx.x & 63);
+  const int j = H + t.j0 + (threadIdx.x >> 6);
+  if (i >= H + a.nx || j >= H + a.ny) return;  // no barriers below
+  const long long N = a.N;
+  const R* W = a.win;
+  // window slot m holds plane k - ZL + m (m = 0..4): the 5-point z stencil of
+  // plane k spans k - FB3 .. k + 3 FB3
+  constexpr int ZL = FB3 > 0 ? 1 : 3;
+  R zw[9][5], nxt[9];
+  unsigned off = cell_off(a, i, j, t.k0);
+  const unsigned pb = a.planeb;
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) zw[v][m] = *at(W + (long long)v * N, off + (unsigned)(m - ZL) * pb);
+    nxt[v] = *at(W + (long long)v * N, off + (unsigned)(4 - ZL) * pb);
+  }
+  for (int k = t.k0; k < t.k1; ++k, off += pb) {
+#pragma unroll
+    for (int v = 0; v < 9; ++v) zw[v][4] = nxt[v];
+    if (k + 1 < t.k1) {
+#pragma unroll
+      for (int v = 0; v < 9; ++v) nxt[v] = *at(W + (long long)v * N, off + (unsigned)(5 - ZL) * pb);
+    }
+    cell_body<R, TM, FB1, FB2, FB3, STAGE>(
+        a, coef, i, j, k, off,
+        [&](int v) -> R { return mac_off<FB1>(W + (long long)v * N, off, (unsigned)sizeof(R)); },
+        [&](int v) -> R { return mac_off<FB2>(W + (long long)v * N, off, a.rowb); },
+        [&](int v) -> R {
+          return R(FB3) * (R(AF_1) * zw[v][ZL - FB3] + R(AF0) * zw[v][ZL] + R(AF1) * zw[v][ZL + FB3] +
+                           R(AF2) * zw[v][ZL + 2 * FB3] + R(AF3) * zw[v][ZL + 3 * FB3]);
+        },
+        [&](int v) -> R { return zw[v][ZL]; });
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) zw[v][m] = zw[v][m + 1];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -335,7 +478,7 @@ __global__ __launch_bounds__(256) void cg_pair(StageArgs<float> a, const float* 
       if (ic < H || ic >= H + a.nx) continue;
       const long long p = (long long)ic + NXl * j + (long long)k * S;
       cell_body<float, float, FB1, FB2, FB3, STAGE>(
-          a, coef, ic, j, k, [&](int v) -> float { return mac<FB1>(W + (long long)v * N + p, 1); },
+          a, coef, ic, j, k, cell_off(a, ic, j, k), [&](int v) -> float { return mac<FB1>(W + (long long)v * N + p, 1); },
           [&](int v) -> float { return mac<FB2>(W + (long long)v * N + p, NXl); },
           [&](int v) -> float { return mac<FB3>(W + (long long)v * N + p, S); },
           [&](int v) -> float { return W[(long long)v * N + p]; });
@@ -511,7 +654,7 @@ __global__ __launch_bounds__(64 * TY) void cg_zcol(StageArgs<R> a, const TM* __r
     if (active) {
       const R* L = &sw[buf][0][0];
       cell_body<R, TM, FB1, FB2, FB3, STAGE>(
-          a, coef, i, j, k,
+          a, coef, i, j, k, cell_off(a, i, j, k),
           [&](int v) -> R {
             const R* f = L + v * T::LP + cidx;
             return R(FB1) * (R(AF_1) * f[-FB1] + R(AF0) * f[0] + R(AF1) * f[FB1] + R(AF2) * f[2 * FB1] +
@@ -570,7 +713,7 @@ __global__ __launch_bounds__(64 * TY) void cg_plane(StageArgs<R> a, const TM* __
   __syncthreads();
   if (!active) return;
   cell_body<R, TM, FB1, FB2, FB3, STAGE>(
-      a, coef, i, j, k,
+      a, coef, i, j, k, cell_off(a, i, j, k),
       [&](int v) -> R {
         const R* f = L + v * T::LP + cidx;
         return R(FB1) * (R(AF_1) * f[-FB1] + R(AF0) * f[0] + R(AF1) * f[FB1] + R(AF2) * f[2 * FB1] +
@@ -603,15 +746,17 @@ __global__ __launch_bounds__(256) void cg_surface(StageArgs<R> a, const TM* __re
   const long long N = a.N, S = a.S, NXl = a.NX;
   const long long p = (long long)i + NXl * j + (long long)k * S;
   const R* W = a.win;
-  auto cf = [&](int c, long long q) -> R { return R(Store<TM>::ld(coef + (long long)c * N + q)); };
+  // q lies in layer k (same-plane neighbours) unless kq says otherwise
+  auto cfk = [&](int c, long long q, int kq) -> R { return coef_at<R, TM, false>(coef, a, c, q, kq); };
+  auto cf = [&](int c, long long q) -> R { return cfk(c, q, k); };
   const R mu = cf(M_MU, p), lam = cf(M_LAM, p), bu = cf(M_B, p);
   const PmlArgs<R>& P = a.pml;
   const R bx = P.beta[0][i], by = P.beta[1][j];
   const int ks = ktop - k + H;  // 3, 4, 5: index of the surface in the 7-point image window
 
   // J * (m . T) at cell q for metric set c0 (0 = xi, 3 = et, 6 = zt)
-  auto jt = [&](long long q, int c0, R* o) {
-    R m1 = cf(c0, q), m2 = cf(c0 + 1, q), m3 = cf(c0 + 2, q), jac = s.jac[q];
+  auto jt = [&](long long q, int kq, int c0, R* o) {
+    R m1 = cfk(c0, q, kq), m2 = cfk(c0 + 1, q, kq), m3 = cfk(c0 + 2, q, kq), jac = s.jac[q];
     R txx = W[3 * N + q], tyy = W[4 * N + q], tzz = W[5 * N + q];
     R txy = W[6 * N + q], txz = W[7 * N + q], tyz = W[8 * N + q];
     o[0] = (m1 * txx + m2 * txy + m3 * txz) * jac;
@@ -625,20 +770,20 @@ __global__ __launch_bounds__(256) void cg_surface(StageArgs<R> a, const TM* __re
   for (int t = 0; t < 5; ++t) {
     int off1 = (t - 1) * FB1, off2 = (t - 1) * FB2;
     R o[3];
-    jt(p + off1, 0, o);
+    jt(p + off1, k, 0, o);
 #pragma unroll
     for (int c = 0; c < 3; ++c) gx[c] += cw[t] * o[c];
-    jt(p + (long long)off2 * NXl, 3, o);
+    jt(p + (long long)off2 * NXl, k, 3, o);
 #pragma unroll
     for (int c = 0; c < 3; ++c) gy[c] += cw[t] * o[c];
     // z: image about the surface, J T = 0 on it
     int l = 3 + (t - 1) * FB3;
     if (l < ks) {
-      jt(p + (long long)(l - 3) * S, 6, o);
+      jt(p + (long long)(l - 3) * S, k + (l - 3), 6, o);
 #pragma unroll
       for (int c = 0; c < 3; ++c) gz[c] += cw[t] * o[c];
     } else if (l > ks) {
-      jt(p + (long long)(2 * ks - l - 3) * S, 6, o);
+      jt(p + (long long)(2 * ks - l - 3) * S, k + (2 * ks - l - 3), 6, o);
 #pragma unroll
       for (int c = 0; c < 3; ++c) gz[c] -= cw[t] * o[c];
     }

@@ -32,6 +32,22 @@ template <> struct Store<__hip_bfloat16> {
   static __device__ __forceinline__ float ld(const __hip_bfloat16* p) { return __bfloat162float(*p); }
 };
 
+// 16-bit coefficient storage (fp16 / bf16): the twelve coefficient planes are
+// stored in pairs (c = 2q, 2q + 1) as one 32-bit word per cell (code of plane
+// 2q in the low half), and each 16-bit value is an affine code over its
+// (plane, k) layer: value = mid[c][k] + half[c][k] * code, code in [-1, 1],
+// with (mid, half) the centre and half-width of the layer's value range in a
+// float table aff[c][k][2]. A layer whose values are all equal is stored
+// exactly; otherwise the rounding error is bounded by half * 2^-11 (fp16) or
+// half * 2^-8 (bf16) instead of |value| times that.
+template <class T> struct Bits16;
+template <> struct Bits16<__half> {
+  static __device__ __forceinline__ float to_float(unsigned short b) { return __half2float(__ushort_as_half(b)); }
+};
+template <> struct Bits16<__hip_bfloat16> {
+  static __device__ __forceinline__ float to_float(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+};
+
 // Coefficient planes, SoA over the padded grid. Metrics are stored as the
 // unit-free products m * DH (i.e. divided by rDH) only when `scaled` is set;
 // the default keeps the reference meaning (xi_x ... zt_z in 1/m).

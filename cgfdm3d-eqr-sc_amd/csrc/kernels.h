@@ -10,12 +10,16 @@ struct StageArgs {
   int NX, NY;           // padded x/y extents
   long long N;          // padded cells per field
   long long S;          // NX * NY
+  unsigned rowb, planeb;  // NX and S in bytes of one grid element (R); N * sizeof(R) < 4 GiB
   int kbeg, kend;       // k range of the interior kernel [kbeg, kend)
   int stage;            // RK4 stage 0..3
   int rtx, rty;         // interior block order: regions of (64 rtx) x (4 rty) columns per XCD
   int kc;               // > 0: z-marching interior kernel with chunks of kc planes
   int zty;              // z-marching tile height: 8 (512 threads) or 4 (256 threads)
+  int zreg;             // > 0: register z-marching interior kernel with chunks of zreg planes
   R DT, rDH;
+  const float* aff;     // 16-bit coefficient storage: affine table [12][NZ][2] (mid, half); else unused
+  int nzp;              // padded z extent (rows of the affine table)
   const R* win;         // stencil input (9 SoA fields)
   const R* wm;          // RK base (state at the start of the step)
   R* wt;                // RK accumulator

@@ -41,6 +41,16 @@ void plane_stage_ty(const StageArgs<R>& a, const TM* coef, dim3 g, dim3 b, hipSt
   }
 }
 
+template <class R, class TM, int FB1, int FB2, int FB3>
+void zreg_stage(const StageArgs<R>& a, const TM* coef, dim3 g, dim3 b, hipStream_t st) {
+  switch (a.stage) {
+    case 0: cg_zreg<R, TM, FB1, FB2, FB3, 0><<<g, b, 0, st>>>(a, coef); break;
+    case 1: cg_zreg<R, TM, FB1, FB2, FB3, 1><<<g, b, 0, st>>>(a, coef); break;
+    case 2: cg_zreg<R, TM, FB1, FB2, FB3, 2><<<g, b, 0, st>>>(a, coef); break;
+    default: cg_zreg<R, TM, FB1, FB2, FB3, 3><<<g, b, 0, st>>>(a, coef); break;
+  }
+}
+
 template <int FB1, int FB2, int FB3>
 void pair_stage(const StageArgs<float>& a, const float* coef, dim3 g, dim3 b, hipStream_t st) {
   switch (a.stage) {
@@ -122,6 +132,23 @@ int launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, h
   }
   if (a.rty < 1) a.rty = 1;
   if (a.rtx < 1) a.rtx = 1;
+  if (a.zreg > 0) {
+    // register z-marching kernel: 64 x 4 column tiles, chunks of zreg planes
+    const long long nb = zreg_blocks(a.nx, a.ny, nk, a.zreg, a.rtx, a.rty);
+    if (nb > 0x7fffffffLL) return LAUNCH_TOO_LARGE;
+    const dim3 grid((unsigned)nb), block(256);
+    switch (fb_index(fb1, fb2, fb3)) {
+      case 0: zreg_stage<R, TM, -1, -1, -1>(a, coef, grid, block, st); break;
+      case 1: zreg_stage<R, TM, 1, -1, -1>(a, coef, grid, block, st); break;
+      case 2: zreg_stage<R, TM, -1, 1, -1>(a, coef, grid, block, st); break;
+      case 3: zreg_stage<R, TM, 1, 1, -1>(a, coef, grid, block, st); break;
+      case 4: zreg_stage<R, TM, -1, -1, 1>(a, coef, grid, block, st); break;
+      case 5: zreg_stage<R, TM, 1, -1, 1>(a, coef, grid, block, st); break;
+      case 6: zreg_stage<R, TM, -1, 1, 1>(a, coef, grid, block, st); break;
+      default: zreg_stage<R, TM, 1, 1, 1>(a, coef, grid, block, st); break;
+    }
+    return LAUNCH_OK;
+  }
   const long long nb = interior_blocks(a.nx, a.ny, nk, a.rtx, a.rty);
   if (nb > 0x7fffffffLL) return LAUNCH_TOO_LARGE;
   const dim3 grid((unsigned)nb), block(256);

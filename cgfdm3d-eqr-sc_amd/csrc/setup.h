@@ -9,6 +9,12 @@ void launch_metrics(const float* coord, double* met, int nx, int ny, int nz, int
 template <class R, class TM>
 void launch_pack(const double* met, const float* med, TM* coef, R* jac, long long N, double stress_scale,
                  hipStream_t st);
+// 16-bit coefficient storage: affine table aff[12][NZ][2] and the six pair-word
+// planes (see Bits16 in common.h), J as float, and mu / Z as float for the
+// moment-rate scaling of the source cells.
+template <class TM>
+void launch_pack16(const double* met, const float* med, float* aff, unsigned* words, float* jac, float* mu_src, int NX,
+                   int NY, int NZ, double stress_scale, hipStream_t st);
 template <class R>
 void launch_rdz(const double* met, const float* med, R* rdzdx, R* rdzdy, int nx, int ny, int nz, int NX, int NY,
                 long long N, hipStream_t st);

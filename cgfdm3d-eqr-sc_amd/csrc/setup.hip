@@ -96,6 +96,79 @@ __global__ void k_pack<double, double>(const double* __restrict__ met, const flo
   jac[p] = met[9 * N + p];
 }
 
+// Coefficient c of cell p before storage: metrics c < 9, then mu / Z, lambda / Z
+// and Z / rho (stress carried as T / Z).
+__device__ __forceinline__ double coef_value(const double* __restrict__ met, const float* __restrict__ med, long long N,
+                                             long long p, int c, double Z) {
+  if (c < 9) return met[c * N + p];
+  const double vs = med[p], vp = med[N + p], rho = med[2 * N + p];
+  if (c == M_MU) return rho * vs * vs / Z;
+  if (c == M_LAM) return rho * (vp * vp - 2.0 * vs * vs) / Z;
+  return Z / rho;
+}
+
+// Affine table of the 16-bit storage: block (k, c) reduces plane c over layer k
+// to (mid, half) = centre and half-width of its value range (FP64), rounded
+// to float.
+__global__ void k_coef_range(const double* __restrict__ met, const float* __restrict__ med, float* __restrict__ aff,
+                             long long S, long long N, int NZ, double Z) {
+  __shared__ double smin[256], smax[256];
+  const int k = blockIdx.x, c = blockIdx.y;
+  double lo = 1e300, hi = -1e300;
+  for (long long e = threadIdx.x; e < S; e += blockDim.x) {
+    const double v = coef_value(met, med, N, (long long)k * S + e, c, Z);
+    lo = fmin(lo, v);
+    hi = fmax(hi, v);
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    aff[2 * ((long long)c * NZ + k)] = (float)(0.5 * (smin[0] + smax[0]));
+    aff[2 * ((long long)c * NZ + k) + 1] = (float)(0.5 * (smax[0] - smin[0]));
+  }
+}
+
+template <class TM> __device__ __forceinline__ unsigned short bits16(float x);
+template <> __device__ __forceinline__ unsigned short bits16<__half>(float x) {
+  return __half_as_ushort(__float2half_rn(x));
+}
+template <> __device__ __forceinline__ unsigned short bits16<__hip_bfloat16>(float x) {
+  const __hip_bfloat16 h = __float2bfloat16(x);
+  return *reinterpret_cast<const unsigned short*>(&h);
+}
+
+// 16-bit pair words: word plane q holds the affine codes of planes 2q (low
+// half) and 2q + 1 (high half); J in the wavefield precision.
+template <class TM>
+__global__ void k_pack16(const double* __restrict__ met, const float* __restrict__ med, const float* __restrict__ aff,
+                         unsigned* __restrict__ words, float* __restrict__ jac, float* __restrict__ mu_src, int NX,
+                         long long S, long long N, int NZ, double Z) {
+  // block (x, j, k): cells i = 256 x + thread of row j in layer k
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= NX) return;
+  const long long p = (long long)k * S + (long long)j * NX + i;
+  for (int q = 0; q < M_NCOEF / 2; ++q) {
+    unsigned short b[2];
+    for (int h = 0; h < 2; ++h) {
+      const int c = 2 * q + h;
+      const double mid = aff[2 * ((long long)c * NZ + k)], half = aff[2 * ((long long)c * NZ + k) + 1];
+      const double v = coef_value(met, med, N, p, c, Z);
+      b[h] = bits16<TM>(half > 0.0 ? (float)((v - mid) / half) : 0.0f);
+    }
+    words[q * N + p] = (unsigned)b[0] | ((unsigned)b[1] << 16);
+  }
+  jac[p] = (float)met[9 * N + p];
+  mu_src[p] = (float)coef_value(met, med, N, p, M_MU, Z);
+}
+
 // Traction-free condition: dV/dzeta = -DZ^-1 (DX dV/dxi + DY dV/deta) on the
 // surface (contravariant.cpp:363-447), with the 3x3 inverse in FP64.
 template <class R>
@@ -386,6 +459,19 @@ template <class R>
 void launch_pack_cmm(const double* cov, const float* med, R* cjm, int NX, int NY, int NZ, long long N, hipStream_t st) {
   k_pack_cmm<R><<<(unsigned)((N + 255) / 256), 256, 0, st>>>(cov, med, cjm, NX, NY, NZ, N);
 }
+
+template <class TM>
+void launch_pack16(const double* met, const float* med, float* aff, unsigned* words, float* jac, float* mu_src, int NX,
+                   int NY, int NZ, double stress_scale, hipStream_t st) {
+  const long long S = (long long)NX * NY, N = S * NZ;
+  k_coef_range<<<dim3(NZ, M_NCOEF), 256, 0, st>>>(met, med, aff, S, N, NZ, stress_scale);
+  k_pack16<TM><<<dim3((NX + 255) / 256, NY, NZ), 256, 0, st>>>(met, med, aff, words, jac, mu_src, NX, S, N, NZ,
+                                                                stress_scale);
+}
+template void launch_pack16<__half>(const double*, const float*, float*, unsigned*, float*, float*, int, int, int,
+                                    double, hipStream_t);
+template void launch_pack16<__hip_bfloat16>(const double*, const float*, float*, unsigned*, float*, float*, int, int,
+                                            int, double, hipStream_t);
 
 void launch_hrange(const float* coord, double* part, int nblocks, int nx, int ny, int nz, int NX, int NY, long long N,
                    hipStream_t st) {

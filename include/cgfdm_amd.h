@@ -64,6 +64,34 @@ typedef struct cgfdm_desc {
  * medium: float[3][NZ][NY][NX] (Vs, Vp, rho, as constructMedium, medium.cpp:251-317) */
 int cgfdm_create(const cgfdm_desc* desc, const float* coord, const float* medium, cgfdm_solver** out);
 
+/* The reference's setup from its input files (run.cpp:28-138 before the time loop):
+ * params.json (getParams.cpp:17-242, NULL = "params.json"), station.json
+ * (station.cpp:92-301, NULL = "station.json"), the grid (flat, Gaussian hill or the
+ * SRTM terrain-following grid, coord.cpp, terrain.cpp:206-860), the medium
+ * (homogeneous, layered, Shen or CRUST1.0: medium.cpp, dealMedium.cpp,
+ * crustMedium.cpp), the source (finite-fault file, multiSource.cpp:780-868, or a
+ * smoothed single source, singleSource.cpp), the stations, and the setup files
+ * the reference writes (model slices data_io.cpp:589-618, terrain and lon/lat
+ * terrain.cpp:28-118, source files multiSource.cpp:594-672) into params.OUT.
+ * nsteps >= 0 replaces NT = TMAX / DT. Errors: 100 no params.json, 1001 parse,
+ * 110 CFL, 130 PML, 180 source. */
+int cgfdm_create_from_params(const char* params_path, const char* stations_path, int scheme, int precision,
+                             int coef_storage, int nsteps, cgfdm_solver** out);
+
+/* Output files of a solver made by cgfdm_create_from_params, in the reference's
+ * byte layouts and rank split: CGFDM_OUT_SLICES the Vx, Vy, Vz slices of the
+ * current state named by the last step taken, plus the free-surface velocity and
+ * displacement slices when sliceFreeSurf is set (data_io.cpp:485-633);
+ * CGFDM_OUT_STATIONS station_mpi_X_Y_Z.bin (station.cpp:422-443); CGFDM_OUT_PGV
+ * PGV_Z_mpi_X_Y_Z.bin (PGV.cpp:177-185). */
+#define CGFDM_OUT_SLICES 1
+#define CGFDM_OUT_STATIONS 2
+#define CGFDM_OUT_PGV 4
+int cgfdm_write_outputs(cgfdm_solver* s, int what);
+/* Step count (NT, or the nsteps given) and IT_SKIP of a solver made by
+ * cgfdm_create_from_params (propagate.cpp:88, the slice cadence). */
+int cgfdm_run_info(cgfdm_solver* s, int* nt, int* it_skip);
+
 /* Finite-fault moment-rate table after solveMomentRate (multiSource.cpp:450-535):
  * rate[npts][nt][6] = (Mxx,Myy,Mzz,Mxy,Mxz,Myz) * area * slip-rate, cell[npts] = padded
  * linear cell index. Points sharing a cell are merged (multiSource.cpp:537-592), the
@@ -160,6 +188,12 @@ int cgfdm_grid_ranges(int nx, int ny, int nz, const float* coord, const float* m
  * 1 .. 2^30 and at most 2^40 padded cells, else CGFDM_ERR_ARG. */
 int cgfdm_grid_ranges_noalloc(int nx, int ny, int nz, const float* coord, const float* medium, double* out8,
                               void* stream);
+
+/* Name of the kernel the solver launches (no reference counterpart; for profiles
+ * and reports): which = 0 the interior stage kernel (CGFDM: cg_cell, cg_zreg, ...)
+ * or the face-flux kernel (AWENO), 1 the surface kernel (CGFDM) or the update
+ * kernel (AWENO). NULL for a bad handle or index. */
+const char* cgfdm_kernel_name(cgfdm_solver* s, int which);
 
 /* Device bytes held by the solver. */
 long long cgfdm_device_bytes(cgfdm_solver* s);

@@ -35,6 +35,10 @@ int fakehip_sticky(void) { return g_sticky; }
 
 void** __hipRegisterFatBinary(const void* data) { static void* h[4]; (void)data; return h; }
 void __hipUnregisterFatBinary(void** h) { (void)h; }
+void __hipRegisterVar(void** modules, void* var, char* hostVar, const char* deviceVar, int ext, size_t size, int constant,
+                      int global) {
+  (void)modules; (void)var; (void)hostVar; (void)deviceVar; (void)ext; (void)size; (void)constant; (void)global;
+}
 void __hipRegisterFunction(void** modules, const void* hostFunction, char* deviceFunction, const char* deviceName,
                            unsigned int threadLimit, void* tid, void* bid, dim3* blockDim, dim3* gridDim,
                            int* wSize) {
@@ -84,6 +88,11 @@ hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t st) { (void)st; 
 hipError_t hipStreamSynchronize(hipStream_t st) { (void)st; return g_sticky; }
 hipError_t hipGetLastError(void) { hipError_t e = g_last; g_last = 0; return e; }
 const char* hipGetErrorString(hipError_t e) { return e == 700 ? "an illegal memory access was encountered (interpreted)" : "fakehip error"; }
+hipError_t hipGetDevice(int* d) { *d = 0; return 0; }
+/* device symbols are not modelled: the allocation-free entry point is not interpreted */
+hipError_t hipGetSymbolAddress(void** p, const void* sym) { (void)sym; *p = 0; return 801; }
+/* no free-memory figure: optional stores (the adjoint's stage states) are not taken */
+hipError_t hipMemGetInfo(size_t* fre, size_t* tot) { *fre = 0; *tot = 0; return 0; }
 hipError_t hipEventCreate(hipEvent_t* e) { *e = (void*)1; return 0; }
 hipError_t hipEventDestroy(hipEvent_t e) { (void)e; return 0; }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t st) { (void)e; (void)st; return 0; }

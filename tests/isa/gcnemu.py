@@ -780,6 +780,10 @@ class Kernel:
             return x & np.uint64(M32)
 
         a = src(1, mods.get("src0_sel", "DWORD"))
+        if base == "v_cvt_f32_f16":  # unary: half in the selected word
+            h = (a & np.uint64(0xFFFF)).astype(np.uint16).view(np.float16)
+            w.vset(o[0], 1, u32f(h.astype(np.float32)), act)
+            return None
         b = src(2, mods.get("src1_sel", "DWORD"))
         r = self.int_op(base, a, b)
         w.vset(o[0], 1, r, act)
@@ -838,6 +842,24 @@ class Kernel:
             w.vset(o[0], 1, ((V(o[1], 1) << (V(o[2], 1) & np.uint64(31))) | V(o[3], 1)) & np.uint64(M32), act)
         elif op == "v_and_or_b32":
             w.vset(o[0], 1, (V(o[1], 1) & V(o[2], 1)) | V(o[3], 1), act)
+        elif op == "v_perm_b32":
+            # byte select from {src0 (bytes 4..7), src1 (bytes 0..3)}; 8..11 sign
+            # fills, 12 zero, 13.. 0xff (CDNA ISA, V_PERM_B32)
+            s0, s1, sel = V(o[1], 1), V(o[2], 1), V(o[3], 1)
+            comb = (s0 << np.uint64(32)) | s1
+            res = np.zeros(NL, np.uint64)
+            for b in range(4):
+                sb = (sel >> np.uint64(8 * b)) & np.uint64(0xFF)
+                byte = (comb >> (np.uint64(8) * np.minimum(sb, np.uint64(7)))) & np.uint64(0xFF)
+                ff = np.uint64(0xFF)
+                byte = np.where(sb == 8, ((s1 >> np.uint64(15)) & np.uint64(1)) * ff, byte)
+                byte = np.where(sb == 9, ((s1 >> np.uint64(31)) & np.uint64(1)) * ff, byte)
+                byte = np.where(sb == 10, ((s0 >> np.uint64(15)) & np.uint64(1)) * ff, byte)
+                byte = np.where(sb == 11, ((s0 >> np.uint64(31)) & np.uint64(1)) * ff, byte)
+                byte = np.where(sb == 12, np.uint64(0), byte)
+                byte = np.where(sb >= 13, ff, byte)
+                res |= byte << np.uint64(8 * b)
+            w.vset(o[0], 1, res, act)
         elif op == "v_not_b32_e32":
             w.vset(o[0], 1, ~V(o[1], 1) & np.uint64(M32), act)
         elif op == "v_lshl_add_u64":

@@ -243,3 +243,105 @@ def test_driver_precision_options(extra, tol, tmp_path, oracle_mod):
         for fam in (slice(0, 3), slice(3, 9)):
             # the files hold float32 samples: the FP64 run agrees to float rounding
             assert rel_l2(got[(i, j, k)][:, fam], so[s_, :, fam]) < max(tol, 1e-6), (extra, s_, fam)
+
+
+def read_station_files(out, nt):
+    got = {}
+    for X in range(PXYZ[0]):
+        for Y in range(PXYZ[1]):
+            for Z in range(PXYZ[2]):
+                f = out / f"station_mpi_{X}_{Y}_{Z}.bin"
+                if not f.exists():
+                    continue
+                raw = f.read_bytes()
+                k = len(raw) // (12 + 4 * nt * 9)
+                xyz = np.frombuffer(raw[:12 * k], np.int32).reshape(k, 3)
+                wave = np.frombuffer(raw[12 * k:], np.float32).reshape(k, nt, 9)
+                for q in range(k):
+                    got[tuple(xyz[q])] = wave[q]
+    return got
+
+
+def srtm_tile(tdir):
+    """A 5-degree SRTM tile (100-105 E, 35-40 N, 6000 x 6000 floats): a gentle ramp
+    with latitude plus a smooth 300 m hill of 1 km width under the grid centre."""
+    P = 6000
+    J = np.arange(P, dtype=np.float32)
+    tile = np.repeat((100.0 + 2000.0 * J / (P - 1))[:, None], P, axis=1).astype(np.float32)
+    r0, c0 = int(round(2.5 / 5.0 * P)), int(round(2.5 / 5.0 * P))
+    rr = np.arange(-60, 61)
+    bump = 300.0 * np.exp(-0.5 * (rr[:, None] ** 2 + rr[None, :] ** 2) / 12.0 ** 2)
+    tile[r0 - 60:r0 + 61, c0 - 60:c0 + 61] += bump.astype(np.float32)
+    os.makedirs(tdir, exist_ok=True)
+    tile.tofile(tdir / "srtm_35N100E.bin")
+
+
+def shen_files(mdir):
+    nlayer = int((2.0e3 + 3e4) / 100.0)
+    os.makedirs(mdir, exist_ok=True)
+    for a in range(3):
+        for b in range(3):
+            lon, lat = 102.0 + 0.5 * a, 37.0 + 0.5 * b
+            K = np.arange(nlayer + 5)
+            vs = 2.0 + 0.01 * K + 0.3 * (lon - 102.0) + 0.2 * (lat - 37.0)
+            np.savetxt(mdir / f"{lon:.1f}_{lat:.1f}.mod", np.stack([K * 0.1, vs, np.full_like(vs, 0.01)], axis=1),
+                       fmt="%.6f")
+
+
+def crust1_files(cdir):
+    nlon, nlat, nl = 360, 180, 9
+    tops = np.array([0.0, -0.3, -1.3, -1.3, -2.4, -6.0, -7.5, -9.0, -12.0])  # 0.3 km top layer is skipped
+    I = np.arange(nlon)[None, :].repeat(nlat, 0).ravel()
+    base = 2.0 + 0.4 * np.arange(nl)
+    vs = base[None, :] + 0.001 * I[:, None]
+    os.makedirs(cdir, exist_ok=True)
+    for name, arr in (("vs", vs), ("vp", 1.8 * vs), ("rho", 2.0 + 0.1 * np.arange(nl)[None, :] + 0 * I[:, None]),
+                      ("bnds", tops[None, :] + 0 * I[:, None])):
+        np.savetxt(cdir / f"crust1.{name}", arr, fmt="%.5f")
+
+
+REAL_DATA = {
+    # SRTM terrain-following grid with the layered file (layers measured from the local surface)
+    "srtm_terrain": lambda d: (srtm_tile(d / "tiles"), dict(
+        useTerrain=1, SRTM90=1, TerrainDir=str(d / "tiles"), lonStart=100, latStart=35, blockX=1, blockY=1,
+        centerLongitude=102.5, centerLatitude=37.5, gauss_hill=0, **{"Depth(km)": 2.0}))[1],
+    # Shen model medium from .mod profiles on a 0.5-degree grid
+    "shen": lambda d: (shen_files(d / "shen"), dict(
+        LayeredModel=0, ShenModel=1, MLonStart=102.0, MLatStart=37.0, MLonEnd=103.0, MLatEnd=38.0, MLonStep=0.5,
+        MLatStep=0.5, MVeticalStep=100.0, MediumDir=str(d / "shen"), centerLongitude=102.5,
+        centerLatitude=37.5))[1],
+    # CRUST1.0 medium (9-layer 1-degree files)
+    "crust1": lambda d: (crust1_files(d / "crust"), dict(
+        LayeredModel=0, Crust_1Model=1, CrustDir=str(d / "crust"), CrustLonStep=1.0, CrustLatStep=1.0,
+        centerLongitude=102.5, centerLatitude=37.5))[1],
+    # Gaussian-STF double couple (singleSource.cpp:217-359) instead of the Ricker explosion
+    "double_couple": lambda d: {"useSingleSource(ricker)": 0, "useSingleSource(double_couple)": 1,
+                                "strike": 30.0, "dip": 60.0, "rake": 90.0, "Mw": 4.0, "duration": 0.2},
+}
+
+
+@pytest.mark.parametrize("case", list(REAL_DATA))
+def test_driver_real_data_builders_match_oracle(case, tmp_path, oracle_mod):
+    """Synthetic SRTM tile, Shen and CRUST1.0 files and the double-couple single
+    source through the driver on the MI355X; the FP64 oracle steps the grid,
+    medium and source table that the driver's own builders make from the same
+    files: station files within 1e-4 relative L2."""
+    lib = T.host_lib()
+    p, st = setup_case(tmp_path, "cgfdm")
+    p.update(REAL_DATA[case](tmp_path))
+    (tmp_path / "params.json").write_text(json.dumps(p))
+    nt = int(0.3 / float(np.float32(0.008)))
+    run_driver(tmp_path, "cgfdm")
+    m, cls = oracle_model(lib, tmp_path, "cgfdm", oracle_mod, nt)
+    if case == "srtm_terrain":
+        relief = float(np.ptp(m.coord[2, H + NZ_ - 1]))
+        assert relief > 50.0, relief  # the hill is in the grid
+    m.stations = (st + H).astype(np.int32)
+    o = cls(m, prec=8, threads=8)
+    o.step(nt)
+    so = o.stations()
+    got = read_station_files(tmp_path / "out", nt)
+    assert len(got) == len(st)
+    for s_, (i, j, k) in enumerate(st):
+        for fam in (slice(0, 3), slice(3, 9)):
+            assert rel_l2(got[(i, j, k)][:, fam], so[s_, :, fam]) < 1e-4, (case, s_, fam)

@@ -106,17 +106,21 @@ def test_fp32_observers_match_fp64_oracle(oracle_mod, solver_mod):
     assert rel_l2(g.surface_displacement(), o64.surface_dis()[:, H:-H, H:-H]) < 1e-4
 
 
-@pytest.mark.parametrize("coef,tol", [(1, 5e-3), (2, 4e-2)])
+@pytest.mark.parametrize("coef,tol", [(1, 3e-4), (2, 5e-3)])
 def test_16bit_coefficients_vs_fp64_oracle(coef, tol, oracle_mod, solver_mod):
     """FP32 wavefield with FP16 (coef=1) or BF16 (coef=2) storage of the metric and
-    medium planes, stress carried in units of a reference impedance: station
-    seismograms against the FP64 oracle within the coefficient rounding
-    (2^-11, 2^-8 relative) times a small amplification."""
+    medium planes as affine codes over each (plane, k) layer, stress carried in
+    units of a reference impedance: station seismograms against the FP64 oracle
+    within the code rounding (2^-12, 2^-9 of the layer's half-range) times a
+    small amplification (measured: fp16 4e-5 .. 1.4e-4, bf16 2e-4 .. 2.6e-3)."""
     m = make("curvilinear_pml_fs", nt=40)
     g, o64 = run_pair(m, oracle_mod, solver_mod, 4, 40, coef_storage=coef)
     sg, s64 = g.stations(), o64.stations()
+    errs = [rel_l2(sg[:, :, c], s64[:, :, c]) for c in range(9)]
+    print(f"coef storage {coef}: station rel L2 per component {['%.1e' % e for e in errs]}, "
+          f"PGV {rel_l2(g.pgv(), o64.pgv()):.1e}", flush=True)
     for c in range(9):
-        assert rel_l2(sg[:, :, c], s64[:, :, c]) < tol, (c, rel_l2(sg[:, :, c], s64[:, :, c]))
+        assert errs[c] < tol, (c, errs[c])
     assert rel_l2(g.pgv(), o64.pgv()) < tol
 
 

@@ -1,7 +1,8 @@
 """Interior-kernel variants and the tight device layout against the oracle.
 
 The library picks its interior kernel and device layout from environment
-knobs read at cgfdm_create (x-paired, plane-tile and z-marching interior
+knobs read at cgfdm_create (register z-marching with chunks of 5 planes and
+of the whole column, x-paired, plane-tile and LDS z-marching interior
 kernels; CGFDM_ROW_ALIGN=0 for the tight layout). Each variant steps the
 curvilinear PML + free-surface case and must match the FP32 oracle as the
 default does (3e-5 relative L2 per field).
@@ -12,6 +13,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 VARIANTS = [
+    ("cgfdm", "CGFDM_INTERIOR_ZREG", "5"),
+    ("cgfdm", "CGFDM_INTERIOR_ZREG", "64"),
     ("cgfdm", "CGFDM_INTERIOR_KC", "-1"),
     ("cgfdm", "CGFDM_INTERIOR_KC", "1"),
     ("cgfdm", "CGFDM_INTERIOR_KC", "16"),
