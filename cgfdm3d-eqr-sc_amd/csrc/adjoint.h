@@ -17,6 +17,7 @@ struct AdjArgs {
   int nx, ny, nz, NX, NY;
   long long N, S;
   int fs, stage;
+  int rtx, rty;     // block order of passes A and B: XCD regions of (64 rtx) x (4 rty) columns
   R DT, rDH;
   const R* X;       // stage state (W planes), for the medium gradient
   const R* mu;      // adjoint of W after the step

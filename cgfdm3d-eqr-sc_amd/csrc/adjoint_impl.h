@@ -85,9 +85,11 @@ __device__ __forceinline__ R g_of(const AdjArgs<R>& a, long long q) {
 // ---------------------------------------------------------------------------
 template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
 __global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restrict__ coef) {
-  const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int k = H + blockIdx.z;
+  CellTile t;
+  if (!interior_tile(blockIdx.x, a.nx, a.ny, a.nz, a.rtx, a.rty, t)) return;
+  const int i = H + t.i0 + (threadIdx.x & 63);
+  const int j = H + t.j0 + (threadIdx.x >> 6);
+  const int k = H + t.k;
   if (i >= H + a.nx || j >= H + a.ny) return;
   const long long N = a.N, S = a.S;
   const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
@@ -192,15 +194,20 @@ __global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restri
 // ---------------------------------------------------------------------------
 template <class R, int FB1, int FB2, int FB3, int STAGE>
 __global__ __launch_bounds__(256) void cg_adj_b(AdjArgs<R> a) {
-  const int i = H + blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = H + blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int k = H + blockIdx.z;
+  CellTile t;
+  if (!interior_tile(blockIdx.x, a.nx, a.ny, a.nz, a.rtx, a.rty, t)) return;
+  const int i = H + t.i0 + (threadIdx.x & 63);
+  const int j = H + t.j0 + (threadIdx.x >> 6);
+  const int k = H + t.k;
   if (i >= H + a.nx || j >= H + a.ny) return;
   const long long N = a.N, S = a.S;
   const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
   const R cw[5] = {R(AF_1), R(AF0), R(AF1), R(AF2), R(AF3)};
   const int FBv[3] = {FB1, FB2, FB3};
   const long long sv[3] = {1, (long long)a.NX, S};
+  // the surface pass scatters into layers ktop - 5 .. ktop only (7-point image
+  // window of the three surface rows): the accumulator is read and cleared there
+  const bool in_ls = a.fs && k >= H + a.nz - 6;
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
     R l = R(0);
@@ -214,7 +221,7 @@ __global__ __launch_bounds__(256) void cg_adj_b(AdjArgs<R> a) {
       l += R(FBv[ax]) * acc * a.rDH;
     }
     const long long q = (long long)v * N + p;
-    if (a.fs) {
+    if (in_ls) {
       l += a.Ls[q];
       a.Ls[q] = R(0);
     }
@@ -613,8 +620,12 @@ int launch_adj_stage(AdjArgs<R> a, const TM* coef, const R* jac, const R* rdzdx,
                      int fb3, hipStream_t st) {
   if (a.stage < 0 || a.stage > 3) return LAUNCH_BAD_STAGE;
   if (a.nx <= 0 || a.ny <= 0 || a.nz <= 0) return LAUNCH_EMPTY;
-  if ((a.ny + 3) / 4 > 65535 || a.nz > 65535) return LAUNCH_TOO_LARGE;
-  const dim3 grid((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz), block(256);
+  if ((a.ny + 3) / 4 > 65535) return LAUNCH_TOO_LARGE;
+  if (a.rtx < 1) a.rtx = 1;
+  if (a.rty < 1) a.rty = 1;
+  const long long nb = interior_blocks(a.nx, a.ny, a.nz, a.rtx, a.rty);
+  if (nb > 0x7fffffffLL) return LAUNCH_TOO_LARGE;
+  const dim3 grid((unsigned)nb), block(256);
   const dim3 gs((a.nx + 63) / 64, (a.ny + 3) / 4, 3);
   // the surface pass reads the auxiliary adjoint g of this stage, which pass A
   // then overwrites with that of the next stage: surface, A, B in this order

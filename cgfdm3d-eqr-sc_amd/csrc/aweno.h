@@ -14,6 +14,8 @@ struct StageArgs {
   int NX, NY, NZ;      // NX: device row pitch
   int XE;              // padded x extent nx + 6 (NX >= XE in the row-aligned layout)
   long long N, S;
+  unsigned rowb, planeb;  // NX and S in bytes of one R element (every plane array < 4 GiB)
+  int rtx, rty;        // block order: XCD regions of (64 rtx) x (4 rty) columns, k-major
   int stage;           // TVD-RK3 stage 0..2
   R DT, rDH, alpha;    // alpha: Lax-Friedrichs speed (global Vp max)
   const R* cjm;        // 19 coefficient planes

@@ -64,18 +64,29 @@ __device__ __forceinline__ void phys_flux(const R (&u)[9], R n1, R n2, R n3, R m
   f[8] = -((n1 * u[1]) * b + (n2 * u[0]) * b);
 }
 
-// Lax-Friedrichs face flux of direction d at face q+1/2, stored at cell q
-// (wave_deriv_alternative_flux_FD_{x,y,z}).
+using cg::at;
+
+// value of plane c of a grid array at byte offset o (SGPR base + 32-bit VGPR
+// offset addressing, as in the CGFDM stage kernels)
 template <class R>
-__device__ __forceinline__ void face_flux(const StageArgs<R>& a, long long p, long long s, int d) {
+__device__ __forceinline__ R pl(const R* base, long long N, int c, unsigned o) {
+  return *at(base + (long long)c * N, o);
+}
+
+// Lax-Friedrichs face flux of direction d at face q+1/2, stored at cell q
+// (wave_deriv_alternative_flux_FD_{x,y,z}); o is the byte offset of cell q,
+// sb the byte stride of direction d.
+template <class R>
+__device__ __forceinline__ void face_flux(const StageArgs<R>& a, unsigned o, unsigned sb, int d) {
   const long long N = a.N;
-  const R n1 = a.cjm[(3 * d + 0) * N + p], n2 = a.cjm[(3 * d + 1) * N + p], n3 = a.cjm[(3 * d + 2) * N + p];
-  const R mu = a.cjm[C_MU * N + p], lam = a.cjm[C_LAM * N + p], b = a.cjm[C_B * N + p];
+  const R n1 = pl(a.cjm, N, 3 * d + 0, o), n2 = pl(a.cjm, N, 3 * d + 1, o), n3 = pl(a.cjm, N, 3 * d + 2, o);
+  const R mu = pl(a.cjm, N, C_MU, o), lam = pl(a.cjm, N, C_LAM, o), b = pl(a.cjm, N, C_B, o);
   R um[9], up[9];
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
     const R* w = a.win + v * N;
-    const R wm2 = w[p - 2 * s], wm1 = w[p - s], w0 = w[p], wp1 = w[p + s], wp2 = w[p + 2 * s], wp3 = w[p + 3 * s];
+    const R wm2 = *at(w, o - 2 * sb), wm1 = *at(w, o - sb), w0 = *at(w, o), wp1 = *at(w, o + sb),
+            wp2 = *at(w, o + 2 * sb), wp3 = *at(w, o + 3 * sb);
     um[v] = weno5<R>(wm2, wm1, w0, wp1, wp2);
     up[v] = weno5<R>(wp3, wp2, wp1, w0, wm1);
   }
@@ -84,7 +95,7 @@ __device__ __forceinline__ void face_flux(const StageArgs<R>& a, long long p, lo
   phys_flux<R>(up, n1, n2, n3, mu, lam, b, fp);
   R* F = a.F + (long long)(9 * d) * N;
 #pragma unroll
-  for (int v = 0; v < 9; ++v) F[v * N + p] = R(0.5) * (fp[v] + fm[v] - a.alpha * (up[v] - um[v]));
+  for (int v = 0; v < 9; ++v) *at(F + v * N, o) = R(0.5) * (fp[v] + fm[v] - a.alpha * (up[v] - um[v]));
 }
 
 // Face fluxes of the three directions. Thread (i, j, k) over [2, n-3)^3
@@ -94,16 +105,20 @@ __device__ __forceinline__ void face_flux(const StageArgs<R>& a, long long p, lo
 template <class R>
 __global__ void __launch_bounds__(256) aw_faces(StageArgs<R> a) {
   // 64-column tiles start at i = 0 so that each wave's row segment starts on
-  // a cache line in the row-aligned layout; lanes with i < 2 have no face
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int k = 2 + blockIdx.z;
+  // a cache line in the row-aligned layout; lanes with i < 2 have no face.
+  // Tiles cover i in [0, XE - 3), j in [2, NY - 3), k in [2, NZ - 3) in the
+  // XCD-aware order of the CGFDM interior kernel.
+  cg::CellTile t;
+  if (!cg::interior_tile(blockIdx.x, a.XE - 3, a.NY - 5, a.NZ - 5, a.rtx, a.rty, t)) return;
+  const int i = t.i0 + (threadIdx.x & 63);
+  const int j = 2 + t.j0 + (threadIdx.x >> 6);
+  const int k = 2 + t.k;
   if (i < 2 || i >= a.XE - 3 || j >= a.NY - 3) return;
-  const long long p = (long long)i + (long long)a.NX * j + (long long)k * a.S;
+  const unsigned o = (unsigned)i * (unsigned)sizeof(R) + (unsigned)j * a.rowb + (unsigned)k * a.planeb;
   const bool ii = i >= H, ji = j >= H, ki = k >= H;
-  if (ji && ki) face_flux<R>(a, p, 1, 0);
-  if (ii && ki) face_flux<R>(a, p, a.NX, 1);
-  if (ii && ji) face_flux<R>(a, p, a.S, 2);
+  if (ji && ki) face_flux<R>(a, o, (unsigned)sizeof(R), 0);
+  if (ii && ki) face_flux<R>(a, o, a.rowb, 1);
+  if (ii && ji) face_flux<R>(a, o, a.planeb, 2);
 }
 
 template <class R>
@@ -140,12 +155,14 @@ __device__ __forceinline__ R tvd(int stage, R m, R w, R h) {
 // aw_update_top.
 template <class R>
 __global__ void __launch_bounds__(256) aw_update(StageArgs<R> a) {
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int k = blockIdx.z;
+  cg::CellTile t;
+  if (!cg::interior_tile(blockIdx.x, a.XE, a.NY, a.NZ, a.rtx, a.rty, t)) return;
+  const int i = t.i0 + (threadIdx.x & 63);
+  const int j = t.j0 + (threadIdx.x >> 6);
+  const int k = t.k;
   if (i >= a.XE || j >= a.NY) return;
   const long long N = a.N;
-  const long long p = (long long)i + (long long)a.NX * j + (long long)k * a.S;
+  const unsigned o = (unsigned)i * (unsigned)sizeof(R) + (unsigned)j * a.rowb + (unsigned)k * a.planeb;
   const bool interior = i >= H && i < a.XE - H && j >= H && j < a.NY - H && k >= H && k < a.NZ - H;
   const int ktop3 = a.NZ - H - 3;
   if (interior && k >= ktop3) return;
@@ -153,16 +170,17 @@ __global__ void __launch_bounds__(256) aw_update(StageArgs<R> a) {
 #pragma unroll
   for (int v = 0; v < 9; ++v) h[v] = R(0);
   if (interior) {
-    const R jac = R(1.0 / (double)a.cjm[C_J * N + p]);
+    const R jac = R(1.0 / (double)pl(a.cjm, N, C_J, o));
     const R cd = a.DT * a.rDH * jac;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const long long s = d == 0 ? 1 : d == 1 ? (long long)a.NX : a.S;
+      const unsigned sb = d == 0 ? (unsigned)sizeof(R) : d == 1 ? a.rowb : a.planeb;
       const R* F = a.F + (long long)(9 * d) * N;
 #pragma unroll
       for (int v = 0; v < 9; ++v) {
         const R* f = F + v * N;
-        const R f_3 = f[p - 3 * s], f_2 = f[p - 2 * s], f_1 = f[p - s], f0 = f[p], f1 = f[p + s], f2 = f[p + 2 * s];
+        const R f_3 = *at(f, o - 3 * sb), f_2 = *at(f, o - 2 * sb), f_1 = *at(f, o - sb), f0 = *at(f, o),
+                f1 = *at(f, o + sb), f2 = *at(f, o + 2 * sb);
         const R fr = f0 - R(1.0 / 24.0) * d2c<R>(f_2, f_1, f0, f1, f2) + R(7.0 / 5760.0) * d4c<R>(f_2, f_1, f0, f1, f2);
         const R fl = f_1 - R(1.0 / 24.0) * d2c<R>(f_3, f_2, f_1, f0, f1) + R(7.0 / 5760.0) * d4c<R>(f_3, f_2, f_1, f0, f1);
         h[v] += cd * (-(fr - fl));
@@ -170,7 +188,8 @@ __global__ void __launch_bounds__(256) aw_update(StageArgs<R> a) {
     }
   }
 #pragma unroll
-  for (int v = 0; v < 9; ++v) a.wout[v * N + p] = tvd<R>(a.stage, a.wm[v * N + p], a.win[v * N + p], h[v]);
+  for (int v = 0; v < 9; ++v)
+    *at(a.wout + v * N, o) = tvd<R>(a.stage, pl(a.wm, N, v, o), pl(a.win, N, v, o), h[v]);
 }
 
 // The three interior layers under the top: the corrections use 6-point
@@ -441,14 +460,15 @@ __global__ void aw_record(const R* __restrict__ W, const R* __restrict__ cjm, lo
 
 template <class R>
 void launch_faces(const StageArgs<R>& a, hipStream_t st) {
-  const dim3 grid((a.XE - 3 + 63) / 64, (a.NY - 5 + 3) / 4, a.NZ - 5), block(256);
-  aw_faces<R><<<grid, block, 0, st>>>(a);
+  const long long nb = cg::interior_blocks(a.XE - 3, a.NY - 5, a.NZ - 5, a.rtx, a.rty);
+  aw_faces<R><<<dim3((unsigned)nb), dim3(256), 0, st>>>(a);
 }
 
 template <class R>
 void launch_update(const StageArgs<R>& a, int nx, int ny, hipStream_t st) {
-  const dim3 grid((a.XE + 63) / 64, (a.NY + 3) / 4, a.NZ), block(256);
-  aw_update<R><<<grid, block, 0, st>>>(a);
+  const dim3 block(256);
+  const long long nb = cg::interior_blocks(a.XE, a.NY, a.NZ, a.rtx, a.rty);
+  aw_update<R><<<dim3((unsigned)nb), block, 0, st>>>(a);
   const dim3 gtop((nx + 63) / 64, (ny + 3) / 4, 3);
   aw_update_top<R><<<gtop, block, 0, st>>>(a, nx, ny);
 }

@@ -1108,6 +1108,7 @@ struct Solver : SolverBase {
           const int sg = (s % 2 == 0) ? 1 : -1;
           cg::AdjArgs<R> a{};
           a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.NX = NX; a.NY = NY; a.N = N; a.S = S;
+          a.rtx = rtx; a.rty = rty;
           a.fs = d.free_surface ? 1 : 0;
           a.stage = s;
           a.DT = R(d.dt);
@@ -1195,8 +1196,9 @@ struct AwenoSolver : SolverBase {
 
   // interior block order: regions of (64 rtx) x (4 rty) columns per XCD; the
   // environment variables CGFDM_INTERIOR_RTX / _RTY (1..64) override it
-  int rty = 4;
-  int rtx = 1;
+  // block order of the face and update kernels (CGFDM_INTERIOR_RTX / _RTY)
+  int rty = 2;
+  int rtx = 8;
   // interior kernel choice (FP32 wavefield): 0 thread-per-cell, -1 x-paired
   // (FP32 coefficients, even NX), 1 plane tile, > 1 z-marching with chunks of kc
   // planes. CGFDM_INTERIOR_KC (-1..4096) overrides it.
@@ -1252,6 +1254,9 @@ struct AwenoSolver : SolverBase {
     NX = galign ? (tNX + 31) / 32 * 32 : tNX;
     S = (long long)NX * NY;
     N = S * NZ;
+    if ((double)N * (double)sizeof(R) >= 4294967296.0)
+      return fail(CGFDM_ERR_ARG, "grid too large: one field plane array must stay below 4 GiB (" +
+                                     std::to_string(N) + " padded cells)");
     pNX = NX;
     pNY = NY;
     pNZ = NZ;
@@ -1364,6 +1369,10 @@ struct AwenoSolver : SolverBase {
         a.NZ = NZ;
         a.N = N;
         a.S = S;
+        a.rowb = (unsigned)(NX * sizeof(R));
+        a.planeb = (unsigned)(S * sizeof(R));
+        a.rtx = rtx;
+        a.rty = rty;
         a.stage = irk;
         a.DT = R(d.dt);
         a.rDH = R(1.0 / d.dh);

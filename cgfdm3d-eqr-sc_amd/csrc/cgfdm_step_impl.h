@@ -111,20 +111,6 @@ __device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontempora
 template <class T>
 __device__ __forceinline__ void st_stream(T* p, T v) { __builtin_nontemporal_store(v, p); }
 
-// Access at a byte offset from a uniform base pointer: the compiler emits the
-// global load/store form with a 64-bit SGPR base and a 32-bit VGPR offset, so
-// one offset register serves every plane of every grid array (field, RK,
-// coefficient). Every grid plane array spans less than 4 GiB (checked on the
-// host before a launch), so byte offsets of cells fit 32 bits.
-template <class T>
-__device__ __forceinline__ const T* at(const T* base, unsigned boff) {
-  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
-}
-template <class T>
-__device__ __forceinline__ T* at(T* base, unsigned boff) {
-  return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff);
-}
-
 // Byte offset of padded cell (i, j, k) in a grid array of R elements (16-bit
 // coefficient words are 4 bytes, as R = float in that storage).
 template <class R>
@@ -268,42 +254,6 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
 // ---------------------------------------------------------------------------
 // interior kernel: one thread per cell below the surface layers
 // ---------------------------------------------------------------------------
-
-// XCD-aware block order of the interior launch. The hardware hands block b to
-// XCD b % 8, so block b is the (b / 8)-th block that XCD runs. The (x, y) plane
-// is cut into regions of rtx 64-column tiles by rty 4-row tiles; regions are
-// dealt to the XCDs in turn and each XCD walks its regions one after another,
-// plane by plane in k and, within a plane, row of tiles by row of tiles. The
-// x, y and z stencil neighbours of a cell are then read by blocks of the same
-// XCD a few planes earlier or later and come from that XCD's L2 instead of
-// HBM, except across region edges. Returns false for padding blocks.
-struct CellTile {
-  int i0, j0, k;
-};
-__device__ __forceinline__ bool interior_tile(int b, int nx, int ny, int nk, int rtx, int rty, CellTile& t) {
-  const int ntx = (nx + 63) / 64, nty = (ny + 3) / 4;
-  const int nrx = (ntx + rtx - 1) / rtx, nry = (nty + rty - 1) / rty;
-  const int per_plane = rtx * rty;
-  const int per_region = per_plane * nk;
-  const int xcd = b & 7, seq = b >> 3;
-  const int slot = seq / per_region, within = seq - slot * per_region;
-  const int r = slot * 8 + xcd;
-  const int kk = within / per_plane, q = within - kk * per_plane;
-  const int ty = q / rtx, tx = q - ty * rtx;
-  const int rx = r % nrx, ry = r / nrx;
-  const int txg = rx * rtx + tx, tyg = ry * rty + ty;
-  t.i0 = txg * 64;
-  t.j0 = tyg * 4;
-  t.k = kk;
-  return r < nrx * nry && txg < ntx && tyg < nty;
-}
-
-// number of blocks of the interior launch for interior_tile()
-inline long long interior_blocks(int nx, int ny, int nk, int rtx, int rty) {
-  const long long ntx = (nx + 63) / 64, nty = (ny + 3) / 4;
-  const long long nregions = ((ntx + rtx - 1) / rtx) * ((nty + rty - 1) / rty);
-  return 8 * ((nregions + 7) / 8) * (long long)rtx * rty * nk;
-}
 
 template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
 __global__ __launch_bounds__(256) void cg_cell(StageArgs<R> a, const TM* __restrict__ coef) {

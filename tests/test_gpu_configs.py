@@ -84,8 +84,14 @@ def test_config1_full_fp32_vs_fp64_oracle(S, oracle_mod):
     say(f"config1: station velocity rel L2 max {max(errs):.2e} median {np.median(errs):.2e}; "
         f"all fields {rel_l2(sg, so):.2e}; PGV {rel_l2(pg, po):.2e}")
     assert max(errs) < 1e-4
-    for c in range(9):
+    # velocities and the in-plane stresses per component; the traction components
+    # Tzz, Txz, Tyz vanish on the free surface, so their error is measured against
+    # the size of the whole stress record
+    for c in (0, 1, 2, 3, 4, 6):
         assert rel_l2(sg[:, :, c], so[:, :, c]) < 1e-4, c
+    stress = np.linalg.norm(so[:, :, 3:])
+    for c in (5, 7, 8):
+        assert np.linalg.norm(sg[:, :, c] - so[:, :, c]) / stress < 1e-4, c
     assert rel_l2(pg, po) < 1e-4
 
 

@@ -292,7 +292,7 @@ def crust1_files(cdir):
     nlon, nlat, nl = 360, 180, 9
     tops = np.array([0.0, -0.3, -1.3, -1.3, -2.4, -6.0, -7.5, -9.0, -12.0])  # 0.3 km top layer is skipped
     I = np.arange(nlon)[None, :].repeat(nlat, 0).ravel()
-    base = 2.0 + 0.4 * np.arange(nl)
+    base = 1.6 + 0.25 * np.arange(nl)
     vs = base[None, :] + 0.001 * I[:, None]
     os.makedirs(cdir, exist_ok=True)
     for name, arr in (("vs", vs), ("vp", 1.8 * vs), ("rho", 2.0 + 0.1 * np.arange(nl)[None, :] + 0 * I[:, None]),
