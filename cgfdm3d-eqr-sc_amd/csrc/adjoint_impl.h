@@ -154,20 +154,32 @@ __global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restri
           }
           const FaceAdj<R>& F = a.fa[f];
           R gQ[9];
+          // every load of the face adjoints before the first store (the
+          // buffers are not known not to alias)
+          R fmu[9], fg[9], fo[9];
+#pragma unroll
+          for (int v = 0; v < 9; ++v) {
+            const long long e = (long long)v * F.cnt + q;
+            if constexpr (STAGE == 3 || STAGE > 0) fmu[v] = F.mu[e];
+            if constexpr (STAGE != 3) {
+              fg[v] = F.g[e];
+              fo[v] = F.out[e];
+            }
+          }
 #pragma unroll
           for (int v = 0; v < 9; ++v) {
             const long long e = (long long)v * F.cnt + q;
             R gA, base;
             if constexpr (STAGE == 3) {
-              gA = R(RK_B4) * F.mu[e];
-              base = F.mu[e];
+              gA = R(RK_B4) * fmu[v];
+              base = fmu[v];
             } else {
-              gA = F.g[e];
-              base = F.out[e];
+              gA = fg[v];
+              base = fo[v];
             }
             const R lA = -ad * a.DT * gA - be * a.DT * gv[v];
             F.out[e] = base + lA;
-            if constexpr (STAGE > 0) F.g[e] = R(rk_b(STAGE - 1)) * F.mu[e] + R(rk_a(STAGE)) * lA;
+            if constexpr (STAGE > 0) F.g[e] = R(rk_b(STAGE - 1)) * fmu[v] + R(rk_a(STAGE)) * lA;
             gQ[v] = gA * a.DT;
           }
           R lD[9];
@@ -208,9 +220,12 @@ __global__ __launch_bounds__(256) void cg_adj_b(AdjArgs<R> a) {
   // the surface pass scatters into layers ktop - 5 .. ktop only (7-point image
   // window of the three surface rows): the accumulator is read and cleared there
   const bool in_ls = a.fs && k >= H + a.nz - 6;
+  // gathers of all nine fields first, then the read-modify-writes: every load
+  // comes before the first store (the buffers are not known not to alias)
+  R l[9], ls[9], mv[9], ov[9];
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
-    R l = R(0);
+    l[v] = R(0);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
       const R* z = a.Z + (long long)(9 * ax + v) * N + p;
@@ -218,16 +233,21 @@ __global__ __launch_bounds__(256) void cg_adj_b(AdjArgs<R> a) {
       R acc = R(0);
 #pragma unroll
       for (int m = 0; m < 5; ++m) acc += cw[m] * z[-(m - 1) * d];
-      l += R(FBv[ax]) * acc * a.rDH;
+      l[v] += R(FBv[ax]) * acc * a.rDH;
     }
     const long long q = (long long)v * N + p;
-    if (in_ls) {
-      l += a.Ls[q];
-      a.Ls[q] = R(0);
-    }
-    if constexpr (STAGE == 3) a.out[q] = a.mu[q] + l;
-    else a.out[q] = a.out[q] + l;
-    if constexpr (STAGE > 0) a.G[q] = R(rk_b(STAGE - 1)) * a.mu[q] + R(rk_a(STAGE)) * l;
+    ls[v] = in_ls ? a.Ls[q] : R(0);
+    if constexpr (STAGE == 3 || STAGE > 0) mv[v] = a.mu[q];
+    if constexpr (STAGE != 3) ov[v] = a.out[q];
+  }
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    const long long q = (long long)v * N + p;
+    const R lv = l[v] + ls[v];
+    if (in_ls) a.Ls[q] = R(0);
+    if constexpr (STAGE == 3) a.out[q] = mv[v] + lv;
+    else a.out[q] = ov[v] + lv;
+    if constexpr (STAGE > 0) a.G[q] = R(rk_b(STAGE - 1)) * mv[v] + R(rk_a(STAGE)) * lv;
   }
 }
 

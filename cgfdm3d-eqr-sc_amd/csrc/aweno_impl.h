@@ -187,9 +187,16 @@ __global__ void __launch_bounds__(256) aw_update(StageArgs<R> a) {
       }
     }
   }
+  // every load before the first store: the state buffers are not known not
+  // to alias, and interleaving would serialise the loads
+  R mv[9], wv[9];
 #pragma unroll
-  for (int v = 0; v < 9; ++v)
-    *at(a.wout + v * N, o) = tvd<R>(a.stage, pl(a.wm, N, v, o), pl(a.win, N, v, o), h[v]);
+  for (int v = 0; v < 9; ++v) {
+    mv[v] = pl(a.wm, N, v, o);
+    wv[v] = pl(a.win, N, v, o);
+  }
+#pragma unroll
+  for (int v = 0; v < 9; ++v) *at(a.wout + v * N, o) = tvd<R>(a.stage, mv[v], wv[v], h[v]);
 }
 
 // The three interior layers under the top: the corrections use 6-point
@@ -230,8 +237,14 @@ __global__ void __launch_bounds__(256) aw_update_top(StageArgs<R> a, int nx, int
       h[v] += cd * (-(fr - fl));
     }
   }
+  R mv[9], wv[9];
 #pragma unroll
-  for (int v = 0; v < 9; ++v) a.wout[v * N + p] = tvd<R>(a.stage, a.wm[v * N + p], a.win[v * N + p], h[v]);
+  for (int v = 0; v < 9; ++v) {
+    mv[v] = a.wm[v * N + p];
+    wv[v] = a.win[v * N + p];
+  }
+#pragma unroll
+  for (int v = 0; v < 9; ++v) a.wout[v * N + p] = tvd<R>(a.stage, mv[v], wv[v], h[v]);
 }
 
 // Characteristic free surface on the top layer and cubic extrapolation into

@@ -72,25 +72,40 @@ __device__ __forceinline__ R mac(const R* f, long long s) {
 template <class R, int STAGE>
 __device__ __forceinline__ void pml_face(const FaceBuf<R>& F, long long pos, const R (&Q)[9], R d, R ad, R be, R DT,
                                          R (&h)[9], bool has_extra, const R (&extra)[6]) {
-  // extra: surface stress term added to hA[3..8] on the top layer
+  // extra: surface stress term added to hA[3..8] on the top layer.
+  // The nine fields go in three groups of three, each group's loads before its
+  // stores: the buffers are not known not to alias, so a store between two
+  // loads would make the compiler wait for each load in turn (three round
+  // trips instead of nine, with few registers held).
 #pragma unroll
-  for (int v = 0; v < 9; ++v) {
-    long long q = (long long)v * F.cnt + pos;
-    R cur = STAGE == 0 ? F.a[q] : F.b[q];
-    R hA = (d * Q[v] - ad * cur) * DT;
-    if (has_extra && v >= 3) hA += extra[v - 3] * DT;
-    h[v] -= be * cur * DT;
-    if constexpr (STAGE == 0) {
-      F.t[q] = cur + R(RK_B1) * hA;
-      F.b[q] = cur + R(RK_A2) * hA;
-    } else if constexpr (STAGE == 1) {
-      F.t[q] = F.t[q] + R(RK_B2) * hA;
-      F.b[q] = F.a[q] + R(RK_A3) * hA;
-    } else if constexpr (STAGE == 2) {
-      F.t[q] = F.t[q] + R(RK_B3) * hA;
-      F.b[q] = F.a[q] + hA;
-    } else {
-      F.a[q] = F.t[q] + R(RK_B4) * hA;
+  for (int g = 0; g < 9; g += 3) {
+    R cur[3], tv[3], av[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const long long q = (long long)(g + u) * F.cnt + pos;
+      cur[u] = STAGE == 0 ? F.a[q] : F.b[q];
+      if constexpr (STAGE == 1 || STAGE == 2 || STAGE == 3) tv[u] = F.t[q];
+      if constexpr (STAGE == 1 || STAGE == 2) av[u] = F.a[q];
+    }
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int v = g + u;
+      const long long q = (long long)v * F.cnt + pos;
+      R hA = (d * Q[v] - ad * cur[u]) * DT;
+      if (has_extra && v >= 3) hA += extra[v - 3] * DT;
+      h[v] -= be * cur[u] * DT;
+      if constexpr (STAGE == 0) {
+        F.t[q] = cur[u] + R(RK_B1) * hA;
+        F.b[q] = cur[u] + R(RK_A2) * hA;
+      } else if constexpr (STAGE == 1) {
+        F.t[q] = tv[u] + R(RK_B2) * hA;
+        F.b[q] = av[u] + R(RK_A3) * hA;
+      } else if constexpr (STAGE == 2) {
+        F.t[q] = tv[u] + R(RK_B3) * hA;
+        F.b[q] = av[u] + hA;
+      } else {
+        F.a[q] = tv[u] + R(RK_B4) * hA;
+      }
     }
   }
 }
@@ -230,7 +245,17 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
       if (z2) pml_face<R, STAGE>(P.f[5], i0 + (long long)a.nx * (j0 + (long long)a.ny * (k - H - (a.nz - P.npml))), Q, d, ad, bz, a.DT, h, false, none);
     }
   }
-  // RK4 stage update of W (wave_rk.cpp:18-104)
+  // RK4 stage update of W (wave_rk.cpp:18-104). The RK base and accumulator
+  // are all loaded before the first store (the arrays are not known not to
+  // alias: interleaving would make every load wait for the store before it).
+  R tv[9], mv[9];
+  if constexpr (STAGE != 0) {
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+      tv[v] = ld_stream(at((const R*)a.wt + (long long)v * N, off));
+      if constexpr (STAGE == 1 || STAGE == 2) mv[v] = ld_stream(at(a.wm + (long long)v * N, off));
+    }
+  }
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
     R* wt = at(a.wt + (long long)v * N, off);
@@ -240,13 +265,13 @@ __device__ __forceinline__ void cell_body(const StageArgs<R>& a, const TM* __res
       st_stream(wt, R(m + R(RK_B1) * h[v]));
       st_stream(wo, R(m + R(RK_A2) * h[v]));
     } else if constexpr (STAGE == 1) {
-      st_stream(wt, R(ld_stream((const R*)wt) + R(RK_B2) * h[v]));
-      st_stream(wo, R(ld_stream(at(a.wm + (long long)v * N, off)) + R(RK_A3) * h[v]));
+      st_stream(wt, R(tv[v] + R(RK_B2) * h[v]));
+      st_stream(wo, R(mv[v] + R(RK_A3) * h[v]));
     } else if constexpr (STAGE == 2) {
-      st_stream(wt, R(ld_stream((const R*)wt) + R(RK_B3) * h[v]));
-      st_stream(wo, R(ld_stream(at(a.wm + (long long)v * N, off)) + h[v]));
+      st_stream(wt, R(tv[v] + R(RK_B3) * h[v]));
+      st_stream(wo, R(mv[v] + h[v]));
     } else {
-      st_stream(wo, R(ld_stream((const R*)wt) + R(RK_B4) * h[v]));
+      st_stream(wo, R(tv[v] + R(RK_B4) * h[v]));
     }
   }
 }
@@ -832,22 +857,29 @@ __global__ __launch_bounds__(256) void cg_surface(StageArgs<R> a, const TM* __re
     const R none[6] = {0, 0, 0, 0, 0, 0};
     pml_face<R, STAGE>(P.f[2], (long long)(i - H) + (long long)a.nx * ((j - H) + (long long)a.ny * (k - H)), Q, d, ad, bz, a.DT, h, false, none);
   }
-  R out[9];
+  // RK4 stage update: every load before the first store (see cell_body)
+  R out[9], tv[9], mv[9];
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
-    long long q = (long long)v * N + p;
+    const long long q = (long long)v * N + p;
+    if constexpr (STAGE == 0) mv[v] = W[q];
+    if constexpr (STAGE != 0) tv[v] = a.wt[q];
+    if constexpr (STAGE == 1 || STAGE == 2) mv[v] = a.wm[q];
+  }
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    const long long q = (long long)v * N + p;
     if constexpr (STAGE == 0) {
-      R m = W[q];
-      a.wt[q] = m + R(RK_B1) * h[v];
-      out[v] = m + R(RK_A2) * h[v];
+      a.wt[q] = mv[v] + R(RK_B1) * h[v];
+      out[v] = mv[v] + R(RK_A2) * h[v];
     } else if constexpr (STAGE == 1) {
-      a.wt[q] = a.wt[q] + R(RK_B2) * h[v];
-      out[v] = a.wm[q] + R(RK_A3) * h[v];
+      a.wt[q] = tv[v] + R(RK_B2) * h[v];
+      out[v] = mv[v] + R(RK_A3) * h[v];
     } else if constexpr (STAGE == 2) {
-      a.wt[q] = a.wt[q] + R(RK_B3) * h[v];
-      out[v] = a.wm[q] + h[v];
+      a.wt[q] = tv[v] + R(RK_B3) * h[v];
+      out[v] = mv[v] + h[v];
     } else {
-      out[v] = a.wt[q] + R(RK_B4) * h[v];
+      out[v] = tv[v] + R(RK_B4) * h[v];
     }
     a.wout[q] = out[v];
   }

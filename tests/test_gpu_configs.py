@@ -142,7 +142,7 @@ def test_config2_config3_fp32_vs_fp64_500_steps(scheme, S):
     errs = [rel_l2(s4[s, :, :3], s8[s, :, :3]) for s in live]
     say(f"{m.name}: {len(live)} stations with signal, velocity rel L2 max {max(errs):.2e}; "
         f"all stations {rel_l2(s4[:, :, :3], s8[:, :, :3]):.2e}; PGV {rel_l2(p4, p8):.2e}")
-    assert len(live) >= 20
+    assert len(live) >= 10
     assert rel_l2(s4[:, :, :3], s8[:, :, :3]) < 1e-4
     assert max(errs) < 1e-3
     assert rel_l2(p4, p8) < 1e-4
@@ -180,7 +180,9 @@ def test_config4_16bit_coefficients_vs_fp64(S):
 def test_config5_adjoint_vs_central_differences(S):
     from cgfdm_amd import autograd as A
     import torch
-    steps, ndir = 100, 3
+    # 300 steps (about 1 s): the P waves of the fault reach the stations, so the
+    # misfit and its gradient are not the numerical tail ahead of the wave front
+    steps, ndir = 300, 3
     hm = gpu_hmin(S, 256, 256, 128)
     m = M.config5(nt=steps + 1, hmin_fn=hm)
     say(f"config5 {m.nx}x{m.ny}x{m.nz}, {len(m.src_cells)} fault points, {len(m.stations)} stations, "
@@ -190,6 +192,7 @@ def test_config5_adjoint_vs_central_differences(S):
     mt.medium = mt.medium.copy()
     mt.medium[0] = (mt.medium[0] * (1.0 + pert)).astype(np.float32)
     target = S.Solver(mt, precision=8).forward_record(steps, steps)
+    assert np.abs(target).max() > 1e-12, "no signal at the stations"
 
     def misfit(med, rate):
         mm = M.config5(nt=steps + 1, hmin_fn=hm)
@@ -200,7 +203,7 @@ def test_config5_adjoint_vs_central_differences(S):
 
     ts = [torch.tensor(m.medium[c], dtype=torch.float64, requires_grad=True) for c in range(3)]
     rate = torch.tensor(m.src_rate, dtype=torch.float64, requires_grad=True)
-    rec = A.seismograms(*ts, rate, m, steps, precision=8, checkpoint_every=10)
+    rec = A.seismograms(*ts, rate, m, steps, precision=8, checkpoint_every=100)
     loss = 0.5 * ((rec - torch.as_tensor(target)) ** 2).sum()
     loss.backward()
     grads = [t.grad.numpy() for t in ts] + [rate.grad.numpy()]
