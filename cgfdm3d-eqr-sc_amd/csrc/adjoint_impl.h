@@ -641,7 +641,6 @@ int launch_adj_stage(AdjArgs<R> a, const TM* coef, const R* jac, const R* rdzdx,
   if (a.stage < 0 || a.stage > 3) return LAUNCH_BAD_STAGE;
   if (a.nx <= 0 || a.ny <= 0 || a.nz <= 0) return LAUNCH_EMPTY;
   if ((a.ny + 3) / 4 > 65535) return LAUNCH_TOO_LARGE;
-  if (a.rtx < 1) a.rtx = 1;
   if (a.rty < 1) a.rty = 1;
   const long long nb = interior_blocks(a.nx, a.ny, a.nz, a.rtx, a.rty);
   if (nb > 0x7fffffffLL) return LAUNCH_TOO_LARGE;

@@ -374,10 +374,12 @@ struct Solver : SolverBase {
 
   // interior block order: regions of (64 rtx) x (4 rty) columns per XCD; the
   // environment variables CGFDM_INTERIOR_RTX / _RTY (1..64) override it
-  // (rtx 8, rty 2: 512 x 8-column regions, measured fastest on the 512 x 512 x
-  // 256 workload: 4.43 ms per launch against 5.08 ms for 64 x 16 regions)
+  // (rtx 0 = whole rows of tiles, rty 2: 512 x 8-column regions on the 512 x
+  // 512 x 256 workload, measured fastest there: 4.43 ms per launch against 5.08
+  // ms for 64 x 16 regions; partial-width regions leave XCDs idle when the row
+  // of tiles does not divide evenly)
   int rty = 2;
-  int rtx = 8;
+  int rtx = 0;
   // interior kernel choice (FP32 wavefield): 0 thread-per-cell, -1 x-paired
   // (FP32 coefficients, even NX), 1 plane tile, > 1 z-marching with chunks of kc
   // planes. CGFDM_INTERIOR_KC (-1..4096) overrides it.
@@ -395,7 +397,7 @@ struct Solver : SolverBase {
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_RTX")) {
       const int v = std::atoi(e);
-      if (v >= 1 && v <= 64) rtx = v;
+      if (v >= 0 && v <= 64) rtx = v;
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_KC")) {
       const int v = std::atoi(e);
@@ -1196,9 +1198,10 @@ struct AwenoSolver : SolverBase {
 
   // interior block order: regions of (64 rtx) x (4 rty) columns per XCD; the
   // environment variables CGFDM_INTERIOR_RTX / _RTY (1..64) override it
-  // block order of the face and update kernels (CGFDM_INTERIOR_RTX / _RTY)
+  // block order of the face and update kernels (CGFDM_INTERIOR_RTX / _RTY;
+  // rtx 0 = whole rows of tiles)
   int rty = 2;
-  int rtx = 8;
+  int rtx = 0;
   // interior kernel choice (FP32 wavefield): 0 thread-per-cell, -1 x-paired
   // (FP32 coefficients, even NX), 1 plane tile, > 1 z-marching with chunks of kc
   // planes. CGFDM_INTERIOR_KC (-1..4096) overrides it.

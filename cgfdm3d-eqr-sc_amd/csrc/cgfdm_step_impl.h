@@ -316,6 +316,8 @@ struct ZregTile {
 __device__ __forceinline__ bool zreg_tile(int b, int nx, int ny, int kbeg, int kend, int zc, int rtx, int rty,
                                           ZregTile& t) {
   const int ntx = (nx + 63) / 64, nty = (ny + 3) / 4, nch = (kend - kbeg + zc - 1) / zc;
+  rtx = region_extent(rtx, ntx);
+  rty = region_extent(rty, nty);
   const int nrx = (ntx + rtx - 1) / rtx, nry = (nty + rty - 1) / rty;
   const int per_chunk = rtx * rty, per_region = per_chunk * nch;
   const int xcd = b & 7, seq = b >> 3;
@@ -334,6 +336,8 @@ __device__ __forceinline__ bool zreg_tile(int b, int nx, int ny, int kbeg, int k
 
 inline long long zreg_blocks(int nx, int ny, int nk, int zc, int rtx, int rty) {
   const long long ntx = (nx + 63) / 64, nty = (ny + 3) / 4, nch = (nk + zc - 1) / zc;
+  rtx = region_extent(rtx, (int)ntx);
+  rty = region_extent(rty, (int)nty);
   const long long nregions = ((ntx + rtx - 1) / rtx) * ((nty + rty - 1) / rty);
   return 8 * ((nregions + 7) / 8) * (long long)rtx * rty * nch;
 }

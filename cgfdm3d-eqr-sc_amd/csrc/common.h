@@ -90,7 +90,8 @@ __device__ __forceinline__ T* at(T* base, unsigned boff) {
 
 // XCD-aware block order of the interior launch. The hardware hands block b to
 // XCD b % 8, so block b is the (b / 8)-th block that XCD runs. The (x, y) plane
-// is cut into regions of rtx 64-column tiles by rty 4-row tiles; regions are
+// is cut into regions of rtx 64-column tiles by rty 4-row tiles (rtx <= 0 or
+// larger than the row: whole rows, so that no region is a partial column); regions are
 // dealt to the XCDs in turn and each XCD walks its regions one after another,
 // plane by plane in k and, within a plane, row of tiles by row of tiles. The
 // x, y and z stencil neighbours of a cell are then read by blocks of the same
@@ -99,8 +100,12 @@ __device__ __forceinline__ T* at(T* base, unsigned boff) {
 struct CellTile {
   int i0, j0, k;
 };
+__device__ __host__ __forceinline__ int region_extent(int r, int n) { return (r <= 0 || r > n) ? n : r; }
+
 __device__ __forceinline__ bool interior_tile(int b, int nx, int ny, int nk, int rtx, int rty, CellTile& t) {
   const int ntx = (nx + 63) / 64, nty = (ny + 3) / 4;
+  rtx = region_extent(rtx, ntx);  // rtx <= 0: whole rows of tiles
+  rty = region_extent(rty, nty);
   const int nrx = (ntx + rtx - 1) / rtx, nry = (nty + rty - 1) / rty;
   const int per_plane = rtx * rty;
   const int per_region = per_plane * nk;
@@ -120,6 +125,8 @@ __device__ __forceinline__ bool interior_tile(int b, int nx, int ny, int nk, int
 // number of blocks of the interior launch for interior_tile()
 inline long long interior_blocks(int nx, int ny, int nk, int rtx, int rty) {
   const long long ntx = (nx + 63) / 64, nty = (ny + 3) / 4;
+  rtx = region_extent(rtx, (int)ntx);
+  rty = region_extent(rty, (int)nty);
   const long long nregions = ((ntx + rtx - 1) / rtx) * ((nty + rty - 1) / rty);
   return 8 * ((nregions + 7) / 8) * (long long)rtx * rty * nk;
 }

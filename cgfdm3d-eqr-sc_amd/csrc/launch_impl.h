@@ -131,7 +131,6 @@ int launch_interior(StageArgs<R> a, const TM* coef, int fb1, int fb2, int fb3, h
    }
   }
   if (a.rty < 1) a.rty = 1;
-  if (a.rtx < 1) a.rtx = 1;
   if (a.zreg > 0) {
     // register z-marching kernel: 64 x 4 column tiles, chunks of zreg planes
     const long long nb = zreg_blocks(a.nx, a.ny, nk, a.zreg, a.rtx, a.rty);
