@@ -18,6 +18,8 @@ as extra keys of the same line:
   aweno         the ME-AWENO step on the 512 x 512 x 256 basin workload;
   fwd_adjoint   forward + adjoint against the plain forward on the adjoint
                 workload (256 x 256 x 128, 100 steps, checkpoint every 100);
+  fwd_adjoint_2000  the same at the adjoint workload's full length: 2000 steps,
+                checkpoint every 100 (20 segments; the adjoint recomputes 19);
   config4       the 1024 x 1024 x 320 grid at h = 50 m with FP32, fp16 and bf16
                 coefficient storage (throughput; the per-station error against
                 FP64 is in `--config4-error` and the GPU tests).
@@ -140,7 +142,7 @@ def main():
     ap.add_argument("--scheme", choices=["cgfdm", "aweno"], default="cgfdm")
     ap.add_argument("--adjoint", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
-                    help="skip the short forward+adjoint measurement of the default run")
+                    help="skip the forward+adjoint measurements of the default run")
     ap.add_argument("--checkpoint", type=int, default=20)
     ap.add_argument("--adjoint-fd", type=int, default=0, metavar="NDIR",
                     help="with --adjoint: check the gradient against central differences in NDIR random directions")
@@ -252,15 +254,18 @@ def main():
     if not args.no_aweno:
         out["aweno"] = aweno_leg(args)
     if not args.no_adjoint:
-        ma, t_fwd, t_rec, t_adj, ok = measure_adjoint(100, 2, 100)
-        out["fwd_adjoint"] = {
-            "cost_ratio_fwd_adj_vs_fwd": (t_rec + t_adj) / t_fwd,
-            "fwd_adj_gcells": ma.cells * 100 / (t_rec + t_adj) / 1e9,
-            "forward_gcells": ma.cells * 100 / t_fwd / 1e9,
-            "gradients_finite": ok,
-            "workload": f"CGFDM forward+adjoint {ma.nx}x{ma.ny}x{ma.nz} h=100m, 4 hills, 4 layers + 5% random, "
-                        f"PML 20, 16x8 finite fault, {len(ma.stations)} stations, 100 steps, checkpoint every 100",
-        }
+        for key, steps in (("fwd_adjoint", 100), ("fwd_adjoint_2000", 2000)):
+            ma, t_fwd, t_rec, t_adj, ok = measure_adjoint(steps, 2, 100)
+            out[key] = {
+                "cost_ratio_fwd_adj_vs_fwd": (t_rec + t_adj) / t_fwd,
+                "fwd_adj_gcells": ma.cells * steps / (t_rec + t_adj) / 1e9,
+                "forward_gcells": ma.cells * steps / t_fwd / 1e9,
+                "forward_s": t_fwd, "forward_record_s": t_rec, "adjoint_s": t_adj,
+                "gradients_finite": ok,
+                "workload": f"CGFDM forward+adjoint {ma.nx}x{ma.ny}x{ma.nz} h=100m, 4 hills, 4 layers + 5% random, "
+                            f"PML 20, 16x8 finite fault, {len(ma.stations)} stations, {steps} steps, "
+                            f"checkpoint every 100",
+            }
     if not args.no_config4:
         out["config4"] = config4_leg(6, 2)
     print(json.dumps(out), flush=True)

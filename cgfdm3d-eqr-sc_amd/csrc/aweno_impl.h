@@ -71,6 +71,54 @@ __device__ __forceinline__ float weno5<float>(float u1, float u2, float u3, floa
   return fmaf(r1, p1, fmaf(r2, p2, r3 * p3)) * wrcp(sum);
 }
 
+// FP32 WENO5 values of both sides of face q+1/2 from w(q-2 .. q+3):
+// um = weno5(w(q-2), .., w(q+2)) and up = weno5(w(q+3), .., w(q-1)), the same
+// smoothness indicators (times 12) and folded linear weights as weno5<float>.
+// The two sides share the second differences of the windows (q-1, q, q+1) and
+// (q, q+1, q+2) and two of their three candidate values (the parabola through
+// the same three cells at the same face): 4 second differences and 4
+// candidates per face instead of 6 and 6. With A_c, C_c the second and the
+// centred first difference of the window starting at c, the one-sided
+// differences of the indicators are C_c + 2 A_c and 2 A_c - C_c.
+__device__ __forceinline__ void weno5_pair(float wm2, float wm1, float w0, float wp1, float wp2, float wp3, float& um,
+                                           float& up) {
+  const float Am2 = wm2 - 2.0f * wm1 + w0, Am1 = wm1 - 2.0f * w0 + wp1;
+  const float A0 = w0 - 2.0f * wp1 + wp2, A1 = wp1 - 2.0f * wp2 + wp3;
+  const float Cm2 = wm2 - w0, Cm1 = wm1 - wp1, C0 = w0 - wp2, C1 = wp1 - wp3;
+  const float Tm1 = fmaf(13.0f * Am1, Am1, 12e-6f), T0 = fmaf(13.0f * A0, A0, 12e-6f);
+  const float e1 = fmaf(2.0f, A0, C0), e3 = fmaf(2.0f, Am2, -Cm2);
+  const float f1 = fmaf(2.0f, Am1, -Cm1), f3 = fmaf(2.0f, A1, C1);
+  // um: windows (q, q+1, q+2), (q-1, q, q+1), (q-2, q-1, q)
+  const float q1 = fmaf(3.0f * e1, e1, T0);
+  const float q2 = fmaf(3.0f * Cm1, Cm1, Tm1);
+  const float q3 = fmaf(13.0f * Am2, Am2, fmaf(3.0f * e3, e3, 12e-6f));
+  // up: windows (q-1, q, q+1), (q, q+1, q+2), (q+1, q+2, q+3)
+  const float r1q = fmaf(3.0f * f1, f1, Tm1);
+  const float r2q = fmaf(3.0f * C0, C0, T0);
+  const float r3q = fmaf(13.0f * A1, A1, fmaf(3.0f * f3, f3, 12e-6f));
+  // d1 = 5/16 times the candidates shared by both sides, and the one-sided
+  // extrapolations times d3 = 1/16 (d2 = 2 d1)
+  const float PA = 0.1171875f * w0 + 0.234375f * wp1 - 0.0390625f * wp2;   // d1 (3 w0 + 6 wp1 - wp2) / 8
+  const float PB = -0.0390625f * wm1 + 0.234375f * w0 + 0.1171875f * wp1;  // d1 (-wm1 + 6 w0 + 3 wp1) / 8
+  const float Pm = 0.0234375f * wm2 - 0.078125f * wm1 + 0.1171875f * w0;   // d3 (3 wm2 - 10 wm1 + 15 w0) / 8
+  const float Pp = 0.0234375f * wp3 - 0.078125f * wp2 + 0.1171875f * wp1;
+  // weights r_k = 1 / q_k^2 (hardware reciprocal, 1 ulp). A form with one
+  // reciprocal per side (products of the other two squares) measured slower
+  // on the MI355X although the reciprocal costs about 3.4 FMAs there.
+  {
+    const float a1 = __builtin_amdgcn_rcpf(q1 * q1), a2 = __builtin_amdgcn_rcpf(q2 * q2),
+                a3 = __builtin_amdgcn_rcpf(q3 * q3);
+    const float sum = fmaf(0.3125f, a1, fmaf(0.625f, a2, 0.0625f * a3));
+    um = fmaf(a1, PA, fmaf(2.0f * a2, PB, a3 * Pm)) * __builtin_amdgcn_rcpf(sum);
+  }
+  {
+    const float a1 = __builtin_amdgcn_rcpf(r1q * r1q), a2 = __builtin_amdgcn_rcpf(r2q * r2q),
+                a3 = __builtin_amdgcn_rcpf(r3q * r3q);
+    const float sum = fmaf(0.3125f, a1, fmaf(0.625f, a2, 0.0625f * a3));
+    up = fmaf(a1, PB, fmaf(2.0f * a2, PA, a3 * Pp)) * __builtin_amdgcn_rcpf(sum);
+  }
+}
+
 // Conservative elastic flux through a face of cofactor normal (n1, n2, n3)
 // (alternative_flux_FD.cpp:140-158).
 template <class R>
@@ -110,8 +158,12 @@ __device__ __forceinline__ void face_flux(const StageArgs<R>& a, unsigned o, uns
     const R* w = a.win + v * N;
     const R wm2 = *at(w, o - 2 * sb), wm1 = *at(w, o - sb), w0 = *at(w, o), wp1 = *at(w, o + sb),
             wp2 = *at(w, o + 2 * sb), wp3 = *at(w, o + 3 * sb);
-    um[v] = weno5<R>(wm2, wm1, w0, wp1, wp2);
-    up[v] = weno5<R>(wp3, wp2, wp1, w0, wm1);
+    if constexpr (sizeof(R) == 4) {
+      weno5_pair(wm2, wm1, w0, wp1, wp2, wp3, um[v], up[v]);
+    } else {
+      um[v] = weno5<R>(wm2, wm1, w0, wp1, wp2);
+      up[v] = weno5<R>(wp3, wp2, wp1, w0, wm1);
+    }
   }
   R* F = a.F + (long long)(9 * d) * N;
   if constexpr (sizeof(R) == 4) {
@@ -138,21 +190,26 @@ __device__ __forceinline__ void face_flux(const StageArgs<R>& a, unsigned o, uns
 // entry of F keeps the zero it was allocated with.
 template <class R>
 __global__ void __launch_bounds__(256) aw_faces(StageArgs<R> a) {
-  // 64-column tiles start at i = 0 so that each wave's row segment starts on
-  // a cache line in the row-aligned layout; lanes with i < 2 have no face.
-  // Tiles cover i in [0, XE - 3), j in [2, NY - 3), k in [2, NZ - 3) in the
-  // XCD-aware order of the CGFDM interior kernel.
+  // 64-column tiles start at the first interior column i = 3 (on a cache line
+  // in the row-aligned layout) and cover the nx interior columns: ceil(nx/64)
+  // tiles per row, no tile of a few columns. Column 2 has only its x face,
+  // which the lane of column 3 computes as a second face. Tiles cover j in
+  // [2, NY - 3), k in [2, NZ - 3) in the XCD-aware order of the CGFDM interior
+  // kernel.
   cg::CellTile t;
-  if (!cg::interior_tile(blockIdx.x, a.XE - 3, a.NY - 5, a.NZ - 5, a.rtx, a.rty, t)) return;
-  const int i = t.i0 + (threadIdx.x & 63);
+  if (!cg::interior_tile(blockIdx.x, a.XE - 2 * H, a.NY - 5, a.NZ - 5, a.rtx, a.rty, t)) return;
+  const int i = H + t.i0 + (threadIdx.x & 63);
   const int j = 2 + t.j0 + (threadIdx.x >> 6);
   const int k = 2 + t.k;
-  if (i < 2 || i >= a.XE - 3 || j >= a.NY - 3) return;
+  if (i >= a.XE - H || j >= a.NY - 3) return;
   const unsigned o = (unsigned)i * (unsigned)sizeof(R) + (unsigned)j * a.rowb + (unsigned)k * a.planeb;
-  const bool ii = i >= H, ji = j >= H, ki = k >= H;
-  if (ji && ki) face_flux<R>(a, o, (unsigned)sizeof(R), 0);
-  if (ii && ki) face_flux<R>(a, o, a.rowb, 1);
-  if (ii && ji) face_flux<R>(a, o, a.planeb, 2);
+  const bool ji = j >= H, ki = k >= H;
+  if (ji && ki) {
+    face_flux<R>(a, o, (unsigned)sizeof(R), 0);
+    if (i == H) face_flux<R>(a, o - (unsigned)sizeof(R), (unsigned)sizeof(R), 0);
+  }
+  if (ki) face_flux<R>(a, o, a.rowb, 1);
+  if (ji) face_flux<R>(a, o, a.planeb, 2);
 }
 
 template <class R>
@@ -183,18 +240,12 @@ __device__ __forceinline__ R tvd(int stage, R m, R w, R h) {
 
 // Flux difference with the alternative-flux correction in all three
 // directions (cal_du_{x,y,z}, alternative_flux_FD.cpp:1216-1435), fused with
-// the TVD-RK3 update. One thread per padded cell: outside the interior the
+// the TVD-RK3 update, at padded cell (i, j, k): outside the interior the
 // right-hand side is zero (the reference's TVD kernels run over the whole
 // padded arrays). Cells of the three layers under the top are left to
 // aw_update_top.
 template <class R>
-__global__ void __launch_bounds__(256) aw_update(StageArgs<R> a) {
-  cg::CellTile t;
-  if (!cg::interior_tile(blockIdx.x, a.XE, a.NY, a.NZ, a.rtx, a.rty, t)) return;
-  const int i = t.i0 + (threadIdx.x & 63);
-  const int j = t.j0 + (threadIdx.x >> 6);
-  const int k = t.k;
-  if (i >= a.XE || j >= a.NY) return;
+__device__ __forceinline__ void update_cell(const StageArgs<R>& a, int i, int j, int k) {
   const long long N = a.N;
   const unsigned o = (unsigned)i * (unsigned)sizeof(R) + (unsigned)j * a.rowb + (unsigned)k * a.planeb;
   const bool interior = i >= H && i < a.XE - H && j >= H && j < a.NY - H && k >= H && k < a.NZ - H;
@@ -231,6 +282,35 @@ __global__ void __launch_bounds__(256) aw_update(StageArgs<R> a) {
   }
 #pragma unroll
   for (int v = 0; v < 9; ++v) *at(a.wout + v * N, o) = tvd<R>(a.stage, mv[v], wv[v], h[v]);
+}
+
+// One thread per interior column: 64-column tiles start at the first interior
+// column (on a cache line in the row-aligned layout) over all NY x NZ padded
+// rows; the lanes of columns 3..5 and nx..nx+2 also update the ghost column
+// three to their left / right (nx >= 3; one lane per ghost column, since the
+// two sets of lanes are disjoint or, for nx < 6, reach different columns).
+template <class R>
+__global__ void __launch_bounds__(256) aw_update(StageArgs<R> a) {
+  cg::CellTile t;
+  if (!cg::interior_tile(blockIdx.x, a.XE - 2 * H, a.NY, a.NZ, a.rtx, a.rty, t)) return;
+  const int i = H + t.i0 + (threadIdx.x & 63);
+  const int j = t.j0 + (threadIdx.x >> 6);
+  const int k = t.k;
+  if (i >= a.XE - H || j >= a.NY) return;
+  update_cell<R>(a, i, j, k);
+  if (i < 2 * H || i >= a.XE - 2 * H) {
+    // ghost column: no right-hand side
+    const unsigned o = (unsigned)(i < 2 * H ? i - H : i + H) * (unsigned)sizeof(R) + (unsigned)j * a.rowb +
+                       (unsigned)k * a.planeb;
+    R mv[9], wv[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+      mv[v] = pl(a.wm, a.N, v, o);
+      wv[v] = pl(a.win, a.N, v, o);
+    }
+#pragma unroll
+    for (int v = 0; v < 9; ++v) *at(a.wout + v * a.N, o) = tvd<R>(a.stage, mv[v], wv[v], R(0));
+  }
 }
 
 // The three interior layers under the top: the corrections use 6-point
@@ -507,14 +587,14 @@ __global__ void aw_record(const R* __restrict__ W, const R* __restrict__ cjm, lo
 
 template <class R>
 void launch_faces(const StageArgs<R>& a, hipStream_t st) {
-  const long long nb = cg::interior_blocks(a.XE - 3, a.NY - 5, a.NZ - 5, a.rtx, a.rty);
+  const long long nb = cg::interior_blocks(a.XE - 2 * H, a.NY - 5, a.NZ - 5, a.rtx, a.rty);
   aw_faces<R><<<dim3((unsigned)nb), dim3(256), 0, st>>>(a);
 }
 
 template <class R>
 void launch_update(const StageArgs<R>& a, int nx, int ny, hipStream_t st) {
   const dim3 block(256);
-  const long long nb = cg::interior_blocks(a.XE, a.NY, a.NZ, a.rtx, a.rty);
+  const long long nb = cg::interior_blocks(a.XE - 2 * H, a.NY, a.NZ, a.rtx, a.rty);
   aw_update<R><<<dim3((unsigned)nb), block, 0, st>>>(a);
   const dim3 gtop((nx + 63) / 64, (ny + 3) / 4, 3);
   aw_update_top<R><<<gtop, block, 0, st>>>(a, nx, ny);

@@ -201,9 +201,8 @@ struct SolverBase {
   // allocated with spare elements and offset so that the first interior
   // column (i = 3) of every row sits on a 128-byte boundary (with a row pitch
   // that is a multiple of 32 elements). `galign` = false keeps offset 0.
-  // `gfirst` is the column whose rows start on a line: 3 for the CGFDM
-  // kernels (tiles start at the first interior cell), 0 for the ME-AWENO
-  // kernels (tiles cover the ghost columns too).
+  // `gfirst` is the column whose rows start on a line: 3, the first interior
+  // cell, where the 64-column tiles of the stage kernels of both schemes start.
   bool galign = true;
   int gfirst = 3;
   template <class X>
@@ -368,9 +367,13 @@ struct Solver : SolverBase {
   R* adj_src = nullptr;
   R* adj_end = nullptr;
   // stage states X0..X3 of every step of one checkpoint segment (4 x 9N per
-  // step), kept when they fit CGFDM_ADJOINT_STORE_GB (default 64 GiB): the
-  // adjoint then reads them instead of recomputing three stages per step
+  // step), kept when they fit CGFDM_ADJOINT_STORE_GB (default 85 % of the free
+  // device memory after the adjoint's work buffers): the adjoint then reads
+  // them instead of recomputing three stages per step. stages_seg is the
+  // segment whose states the store holds (-1: none): forward_record fills it
+  // with its last segment, which the adjoint then does not recompute.
   R* adj_stages = nullptr;
+  int stages_seg = -1;
 
   // interior block order: regions of (64 rtx) x (4 rty) columns per XCD; the
   // environment variables CGFDM_INTERIOR_RTX / _RTY (1..64) override it
@@ -906,6 +909,39 @@ struct Solver : SolverBase {
     ck_store = rec_vel = adj_buf = adj_face = adj_seg = adj_src = adj_end = adj_stages = nullptr;
     adj_grad = adj_gsrc = nullptr;
     fr_n = 0;
+    stages_seg = -1;
+  }
+  // bytes of the adjoint's work buffers besides the stage store
+  double adjoint_work_bytes(int nsteps, int ck) const {
+    long long fcnt = 0;
+    for (int f = 0; f < 6; ++f) fcnt += 9 * pml.f[f].cnt;
+    const double r = (double)sizeof(R);
+    return r * (72.0 * N + 3.0 * std::max(fcnt, 1LL) + (double)state_size() * (ck + 1) +
+                3.0 * std::max(ns, 1) * nsteps) +
+           8.0 * (3.0 * N + 6.0 * std::max((long long)src_np * src_nt, 1LL));
+  }
+  // allocate the stage store of a ck-step segment if it fits the budget:
+  // CGFDM_ADJOINT_STORE_GB, else 85 % of the free device memory less `reserve`
+  // bytes still to be allocated (an MI355X's 288 GB hold the stage states of a
+  // 100-step segment of a 256 x 256 x 128 grid). The store is optional: on a
+  // failed allocation the runtime's sticky error is cleared so that the launch
+  // checks of the recompute path do not report it.
+  void alloc_stage_store(int ck, double reserve) {
+    if (adj_stages) return;
+    double budget = 0.0;
+    if (const char* e = std::getenv("CGFDM_ADJOINT_STORE_GB")) {
+      budget = std::atof(e) * 1073741824.0;
+    } else {
+      size_t fre = 0, tot = 0;
+      if (hipMemGetInfo(&fre, &tot) == hipSuccess) budget = 0.85 * ((double)fre - reserve);
+      (void)hipGetLastError();
+    }
+    const double need = (double)sizeof(R) * 36.0 * (double)N * ck;
+    if (need <= budget && dalloc(&adj_stages, 36LL * N * ck) != CGFDM_OK) {
+      adj_stages = nullptr;
+      (void)hipGetLastError();
+    }
+    stages_seg = -1;
   }
   // source table index and weight at step index `step` (addMoment.cpp:262-377), -1 if none
   int source_index(int step, float& tw) const {
@@ -920,7 +956,7 @@ struct Solver : SolverBase {
   }
   // one forward step (no observers) that leaves its stage states in x[0..3]:
   // x0 = W after the source, x1..x3 = inputs of stages 1..3
-  int step_store(R* x0, R* x1, R* x2, R* x3) {
+  int step_store(R* x0, R* x1, R* x2, R* x3, bool observe = false) {
     const int kend = cg::H + d.nz - (d.free_surface ? 3 : 0);
     int fb[3] = {FBTAB[it % 8][0], FBTAB[it % 8][1], FBTAB[it % 8][2]};
     float tw = 0.f;
@@ -938,7 +974,7 @@ struct Solver : SolverBase {
       a.win = ins[irk]; a.wout = outs[irk]; a.wm = A; a.wt = T; a.pml = pml;
       LAUNCHCHK((cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream)), "interior stage");
       if (d.free_surface) {
-        cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, nullptr, dis};
+        cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, observe ? pgv : nullptr, dis};
         LAUNCHCHK((cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], stream)), "surface stage");
       }
       fb[0] = -fb[0]; fb[1] = -fb[1]; fb[2] = -fb[2];
@@ -995,11 +1031,21 @@ struct Solver : SolverBase {
     if ((rc = dalloc(&rec_vel, (long long)std::max(ns, 1) * nsteps * 3))) return rc;
     fr_it0 = it;
     fr_ck = ck;
+    // the last segment runs into the stage store when it fits, so that the
+    // adjoint starts from its stage states instead of recomputing them
+    alloc_stage_store(ck, adjoint_work_bytes(nsteps, ck));
+    const int last0 = (nck - 1) * ck;
     for (int n = 0; n < nsteps; ++n) {
       if (n % ck == 0 && (rc = copy_state(ck_store + (long long)(n / ck) * ss, true))) return rc;
-      if ((rc = step_impl(1, true))) return rc;
+      if (adj_stages && n >= last0) {
+        R* xs = adj_stages + 36LL * N * (n - last0);
+        if ((rc = step_store(xs, xs + 9 * N, xs + 18 * N, xs + 27 * N, true))) return rc;
+      } else if ((rc = step_impl(1, true))) {
+        return rc;
+      }
       cg::launch_record_vel<R>(A, N, ns, st_cell, rec_vel, nsteps, n, stream);
     }
+    if (adj_stages) stages_seg = nck - 1;
     HIPCHK(hipGetLastError());
     fr_n = nsteps;
     if (rec && ns > 0) {
@@ -1053,41 +1099,25 @@ struct Solver : SolverBase {
         off += 3 * c;
       }
     }
-    if (!adj_stages) {
-      // budget of the stage store: CGFDM_ADJOINT_STORE_GB, else 85 % of the
-      // device memory still free (288 GB on an MI355X hold the stage states of a
-      // 100-step segment of a 256 x 256 x 128 grid)
-      double budget = 0.0;
-      if (const char* e = std::getenv("CGFDM_ADJOINT_STORE_GB")) {
-        budget = std::atof(e) * 1073741824.0;
-      } else {
-        size_t fre = 0, tot = 0;
-        if (hipMemGetInfo(&fre, &tot) == hipSuccess) budget = 0.85 * (double)fre;
-        (void)hipGetLastError();
-      }
-      const double need = (double)sizeof(R) * 36.0 * (double)N * fr_ck;
-      if (need <= budget && dalloc(&adj_stages, 36LL * N * fr_ck) != CGFDM_OK) {
-        // the stage store is optional: fall back to recomputing the stages, and
-        // clear the runtime's sticky error of the failed allocation so that the
-        // launch checks of the recompute path do not report it
-        adj_stages = nullptr;
-        (void)hipGetLastError();
-      }
-    }
+    alloc_stage_store(fr_ck, 0.0);
     if ((rc = copy_state(adj_end, true))) return rc;
     const int nck = (fr_n + fr_ck - 1) / fr_ck;
     for (int seg = nck - 1; seg >= 0; --seg) {
       const int n0 = seg * fr_ck, n1 = std::min(n0 + fr_ck, fr_n);
-      if ((rc = copy_state(ck_store + (long long)seg * ss, false))) return rc;
-      it = fr_it0 + n0;
-      for (int n = n0; n < n1; ++n) {
-        if (adj_stages) {
-          R* xs = adj_stages + 36LL * N * (n - n0);
-          if ((rc = step_store(xs, xs + 9 * N, xs + 18 * N, xs + 27 * N))) return rc;
-        } else {
-          if ((rc = copy_state(adj_seg + (long long)(n - n0) * ss, true))) return rc;
-          if ((rc = step_impl(1, false))) return rc;
+      if (!(adj_stages && stages_seg == seg)) {
+        if ((rc = copy_state(ck_store + (long long)seg * ss, false))) return rc;
+        it = fr_it0 + n0;
+        stages_seg = -1;
+        for (int n = n0; n < n1; ++n) {
+          if (adj_stages) {
+            R* xs = adj_stages + 36LL * N * (n - n0);
+            if ((rc = step_store(xs, xs + 9 * N, xs + 18 * N, xs + 27 * N))) return rc;
+          } else {
+            if ((rc = copy_state(adj_seg + (long long)(n - n0) * ss, true))) return rc;
+            if ((rc = step_impl(1, false))) return rc;
+          }
         }
+        if (adj_stages) stages_seg = seg;
       }
       for (int n = n1 - 1; n >= n0; --n) {
         cg::launch_adj_inject<R>(MU, N, ns, st_cell, adj_src, fr_n, n, stream);
@@ -1196,17 +1226,11 @@ struct AwenoSolver : SolverBase {
   R* src_rate = nullptr;
   std::vector<double> jac_host;  // FP64 J for the source scaling
 
-  // interior block order: regions of (64 rtx) x (4 rty) columns per XCD; the
-  // environment variables CGFDM_INTERIOR_RTX / _RTY (1..64) override it
-  // block order of the face and update kernels (CGFDM_INTERIOR_RTX / _RTY;
-  // rtx 0 = whole rows of tiles)
+  // block order of the face and update kernels: regions of (64 rtx) x (4 rty)
+  // columns per XCD, rtx 0 = whole rows of tiles; the environment variables
+  // CGFDM_INTERIOR_RTX (0..64) / _RTY (1..64) override it
   int rty = 2;
   int rtx = 0;
-  // interior kernel choice (FP32 wavefield): 0 thread-per-cell, -1 x-paired
-  // (FP32 coefficients, even NX), 1 plane tile, > 1 z-marching with chunks of kc
-  // planes. CGFDM_INTERIOR_KC (-1..4096) overrides it.
-  int kc = 0;
-  int zty = 8;  // z-marching tile height (CGFDM_INTERIOR_ZTY: 4 or 8)
 
   int init(const cgfdm_desc& dd, const float* coord, const float* medium) override {
     d = dd;
@@ -1214,13 +1238,11 @@ struct AwenoSolver : SolverBase {
       const int v = std::atoi(e);
       if (v >= 1 && v <= 64) rty = v;
     }
-    if (const char* e = std::getenv("CGFDM_INTERIOR_KC")) {
+    if (const char* e = std::getenv("CGFDM_INTERIOR_RTX")) {
       const int v = std::atoi(e);
-      if (v >= -1 && v <= 4096) kc = v;
+      if (v >= 0 && v <= 64) rtx = v;
     }
-    if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
     if (const char* e = std::getenv("CGFDM_ROW_ALIGN")) galign = std::atoi(e) != 0;
-    gfirst = 0;
     tNX = d.nx + 2 * cg::H;
     NX = tNX;
     NY = d.ny + 2 * cg::H;
@@ -1520,6 +1542,9 @@ int cgfdm_create(const cgfdm_desc* desc, const float* coord, const float* medium
   std::unique_ptr<SolverBase> s;
   if (d.scheme == CGFDM_SCHEME_AWENO) {
     if (d.coef_storage != 0) return fail(CGFDM_ERR_ARG, "16-bit coefficient storage is a CGFDM option");
+    // the update kernel's lanes of the first and last three interior columns
+    // also update the ghost columns
+    if (d.nx < 3) return fail(CGFDM_ERR_ARG, "the ME-AWENO scheme needs nx >= 3");
     if (d.precision == 8) s.reset(new AwenoSolver<double>());
     else if (d.precision == 4) s.reset(new AwenoSolver<float>());
     else return fail(CGFDM_ERR_ARG, "precision must be 4 or 8");

@@ -951,7 +951,7 @@ class Kernel:
         elif op == "v_div_scale_f64":
             w.vset(o[0], 2, V(o[2], 2, True), act)
             w.sset(o[1], 2, 0)
-        elif op in ("v_add_f32_e32", "v_sub_f32_e32", "v_sub_f32_e64", "v_mul_f32_e64", "v_fmamk_f32",
+        elif op in ("v_add_f32_e32", "v_sub_f32_e32", "v_sub_f32_e64", "v_mul_f32_e64", "v_fmamk_f32", "v_fmaak_f32",
                     "v_rcp_f32_e32", "v_rcp_iflag_f32_e32", "v_exp_f32_e32", "v_trunc_f32_e32", "v_rndne_f32_e32",
                     "v_div_fmas_f32", "v_ldexp_f32", "v_cvt_f32_u32_e32", "v_cvt_f32_i32_e32", "v_cvt_f32_f64_e64"):
             with np.errstate(all="ignore"):
@@ -962,7 +962,8 @@ class Kernel:
                     r = x(1) - x(2)
                 elif op == "v_mul_f32_e64":
                     r = x(1) * x(2)
-                elif op == "v_fmamk_f32":
+                elif op in ("v_fmamk_f32", "v_fmaak_f32"):
+                    # fmamk: src0 * K + src1, fmaak: src0 * src1 + K (operands in that order)
                     r = x(1) * x(2) + x(3)
                 elif op in ("v_rcp_f32_e32", "v_rcp_iflag_f32_e32"):
                     r = 1.0 / x(1)
@@ -1035,6 +1036,9 @@ class Kernel:
             w.vset(o[0], 1, (V(o[1], 1) >> off) & ((np.uint64(1) << wid) - np.uint64(1)), act)
         elif op == "v_sub_u32_e32":
             w.vset(o[0], 1, (V(o[1], 1) - V(o[2], 1)) & np.uint64(M32), act)
+        elif op in ("v_min_i32_e32", "v_max_i32_e32"):
+            f = np.minimum if op == "v_min_i32_e32" else np.maximum
+            w.vset(o[0], 1, f(s32(V(o[1], 1)), s32(V(o[2], 1))).astype(np.int64).astype(np.uint64) & np.uint64(M32), act)
         elif op in ("v_sub_co_u32_e32", "v_subrev_co_u32_e64", "v_add_co_u32_e32"):
             a, b = V(o[2], 1), V(o[3], 1)
             if op == "v_subrev_co_u32_e64":

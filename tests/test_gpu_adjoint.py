@@ -70,6 +70,35 @@ def test_adjoint_leaves_the_forward_state(oracle_mod, solver_mod):
     np.testing.assert_array_equal(a.wavefield(), b.wavefield())
 
 
+@pytest.mark.parametrize("nst,ck", [(10, 3), (6, 6)])
+def test_stage_store_paths_agree(nst, ck, solver_mod, monkeypatch):
+    """forward_record keeps the stage states of its last segment for the
+    adjoint, which then recomputes only the other segments. The gradients equal
+    those of a run without the stage store (every segment and stage
+    recomputed), and a second adjoint call after the same forward_record gives
+    the same gradients (the store then holds the first segment). The surface
+    pass adds with atomics, so agreement is to rounding: 1e-12 of the largest
+    entry in FP64."""
+    m = M.small_config(nt=nst, **CASES["pml_fs_curvilinear"])
+    print(f"stage store: {nst} steps, checkpoint {ck}", flush=True)
+    g = solver_mod.Solver(m, precision=8)
+    rec = g.forward_record(nst, ck)
+    first = g.adjoint(rec)
+    second = g.adjoint(rec)
+    g.close()
+    monkeypatch.setenv("CGFDM_ADJOINT_STORE_GB", "0")
+    h = solver_mod.Solver(m, precision=8)
+    rec0 = h.forward_record(nst, ck)
+    plain = h.adjoint(rec0)
+    h.close()
+    np.testing.assert_array_equal(rec, rec0)
+    for k in ("gmu", "glam", "gb", "grate"):
+        scale = np.abs(plain[k]).max()
+        assert scale > 0, k
+        np.testing.assert_allclose(first[k], plain[k], rtol=0, atol=1e-12 * scale, err_msg=k)
+        np.testing.assert_allclose(second[k], plain[k], rtol=0, atol=1e-12 * scale, err_msg=k)
+
+
 def test_autograd_wrapper_on_the_hip_backend(oracle_mod):
     """torch gradients through the HIP adjoint equal those through the oracle."""
     import torch
