@@ -374,6 +374,7 @@ struct Solver : SolverBase {
   // with its last segment, which the adjoint then does not recompute.
   R* adj_stages = nullptr;
   int stages_seg = -1;
+  int stages_ck = 0;  // segment length the store was allocated for
 
   // interior block order: regions of (64 rtx) x (4 rty) columns per XCD; the
   // environment variables CGFDM_INTERIOR_RTX / _RTY (1..64) override it
@@ -902,11 +903,15 @@ struct Solver : SolverBase {
     }
     return CGFDM_OK;
   }
+  // frees the checkpoints and work buffers of the last forward_record/adjoint;
+  // the stage store stays allocated for the next forward_record with the same
+  // checkpoint interval (it is the largest buffer; its allocation took up to a
+  // few seconds on the MI355X) and is freed with the solver
   void release_adjoint() {
     for (void* q : {(void*)ck_store, (void*)rec_vel, (void*)adj_buf, (void*)adj_face, (void*)adj_grad,
-                    (void*)adj_gsrc, (void*)adj_seg, (void*)adj_src, (void*)adj_end, (void*)adj_stages})
+                    (void*)adj_gsrc, (void*)adj_seg, (void*)adj_src, (void*)adj_end})
       if (q) dfree(q);
-    ck_store = rec_vel = adj_buf = adj_face = adj_seg = adj_src = adj_end = adj_stages = nullptr;
+    ck_store = rec_vel = adj_buf = adj_face = adj_seg = adj_src = adj_end = nullptr;
     adj_grad = adj_gsrc = nullptr;
     fr_n = 0;
     stages_seg = -1;
@@ -927,21 +932,27 @@ struct Solver : SolverBase {
   // failed allocation the runtime's sticky error is cleared so that the launch
   // checks of the recompute path do not report it.
   void alloc_stage_store(int ck, double reserve) {
-    if (adj_stages) return;
-    double budget = 0.0;
-    if (const char* e = std::getenv("CGFDM_ADJOINT_STORE_GB")) {
-      budget = std::atof(e) * 1073741824.0;
-    } else {
+    const double need = (double)sizeof(R) * 36.0 * (double)N * ck;
+    const char* env = std::getenv("CGFDM_ADJOINT_STORE_GB");
+    const double env_budget = env ? std::atof(env) * 1073741824.0 : 0.0;
+    // a store of this size kept from an earlier forward_record
+    if (adj_stages && stages_ck == ck && (!env || need <= env_budget)) return;
+    stages_seg = -1;
+    if (adj_stages) {
+      dfree(adj_stages);
+      adj_stages = nullptr;
+    }
+    double budget = env_budget;
+    if (!env) {
       size_t fre = 0, tot = 0;
       if (hipMemGetInfo(&fre, &tot) == hipSuccess) budget = 0.85 * ((double)fre - reserve);
       (void)hipGetLastError();
     }
-    const double need = (double)sizeof(R) * 36.0 * (double)N * ck;
     if (need <= budget && dalloc(&adj_stages, 36LL * N * ck) != CGFDM_OK) {
       adj_stages = nullptr;
       (void)hipGetLastError();
     }
-    stages_seg = -1;
+    stages_ck = ck;
   }
   // source table index and weight at step index `step` (addMoment.cpp:262-377), -1 if none
   int source_index(int step, float& tw) const {

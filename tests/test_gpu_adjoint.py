@@ -99,6 +99,30 @@ def test_stage_store_paths_agree(nst, ck, solver_mod, monkeypatch):
         np.testing.assert_allclose(second[k], plain[k], rtol=0, atol=1e-12 * scale, err_msg=k)
 
 
+def test_stage_store_reused_by_the_next_forward_record(solver_mod):
+    """The stage store stays allocated from one forward_record to the next with
+    the same checkpoint interval: the second forward_record + adjoint of a
+    solver equals those of a fresh solver started at the same step."""
+    nst, ck = 6, 4
+    m = M.small_config(nt=3 * nst, **CASES["pml_fs_curvilinear"])
+    print(f"stage store reuse: {nst} steps, checkpoint {ck}", flush=True)
+    g = solver_mod.Solver(m, precision=8)
+    g.adjoint(g.forward_record(nst, ck))
+    rec = g.forward_record(nst, ck)
+    grads = g.adjoint(rec)
+    g.close()
+    h = solver_mod.Solver(m, precision=8)
+    h.step(nst)
+    rec_h = h.forward_record(nst, ck)
+    grads_h = h.adjoint(rec_h)
+    h.close()
+    np.testing.assert_array_equal(rec, rec_h)
+    for k in ("gmu", "glam", "gb", "grate"):
+        scale = np.abs(grads_h[k]).max()
+        assert scale > 0, k
+        np.testing.assert_allclose(grads[k], grads_h[k], rtol=0, atol=1e-12 * scale, err_msg=k)
+
+
 def test_autograd_wrapper_on_the_hip_backend(oracle_mod):
     """torch gradients through the HIP adjoint equal those through the oracle."""
     import torch
