@@ -46,6 +46,8 @@ CASES = {
     # without the free surface (PML on the top face then)
     "three_x_tiles": dict(nx=140, ny=16, nz=12, npml=4, hills=2, layered=True, free_surface=True),
     "three_x_tiles_z2": dict(nx=140, ny=16, nz=14, npml=4, hills=0, layered=True, free_surface=False),
+    # nx a multiple of 64: every x tile full
+    "two_full_x_tiles": dict(nx=128, ny=12, nz=12, npml=4, hills=2, layered=True, free_surface=True),
 }
 
 
@@ -135,3 +137,4 @@ def test_error_codes(solver_mod):
     with pytest.raises(solver_mod.CgfdmError) as e:
         solver_mod.Solver(m, precision=4)
     assert e.value.code == 130
+
