@@ -26,7 +26,9 @@ as extra keys of the same line:
                 FP64 is in `--config4-error` and the GPU tests).
 
 `--scheme aweno` measures the ME-AWENO step alone and prints its own line.
-`--adjoint` measures forward + adjoint with its own step count and checkpoint.
+`--adjoint` measures forward + adjoint with its own step count and checkpoint
+(`--adjoint-fd NDIR` checks its gradient against central differences).
+`--aweno-accuracy STEPS` compares FP32 with FP64 on the ME-AWENO basin workload.
 """
 import argparse
 import json
@@ -148,6 +150,8 @@ def main():
     ap.add_argument("--adjoint-fd", type=int, default=0, metavar="NDIR",
                     help="with --adjoint: check the gradient against central differences in NDIR random directions")
     ap.add_argument("--fd-steps", type=int, default=100, help="time steps of the finite-difference check")
+    ap.add_argument("--aweno-accuracy", type=int, default=0, metavar="STEPS",
+                    help="only: ME-AWENO basin workload, FP32 against FP64 over STEPS steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -167,6 +171,8 @@ def main():
         args.cpu_steps = 1 if args.scheme == "aweno" else 3
     if args.copy_calibration:
         return copy_calibration()
+    if args.aweno_accuracy > 0:
+        return aweno_accuracy(args)
     if args.scheme == "aweno":
         return bench_aweno(args)
     if args.adjoint and args.adjoint_fd > 0:
@@ -296,6 +302,45 @@ def accuracy_leg(coef):
     return {"workload": "CGFDM half-space 128x128x64 h=100m, PML 12, double couple at 3 km, 25 receivers, 1000 steps",
             "reference": "FP64 run of the same kernels (FP64 kernels = FP64 oracle to 1e-10 in the parity tests)",
             "station_velocity_rel_l2": l2, "worst_station_rel_l2": worst, "pgv_rel_l2": rel(pa, pb)}
+
+
+def aweno_accuracy(args):
+    """FP32 against FP64 on the ME-AWENO basin workload (512 x 512 x 256, 400
+    stations, PGV) over `--aweno-accuracy` steps: relative L2 of the station
+    velocities (all stations, and the worst station carrying signal) and of the
+    PGV map, and the FP32 step time."""
+    import numpy as np
+    from cgfdm_amd import model as M
+    from cgfdm_amd import solver as S
+    nst = args.aweno_accuracy
+    t0 = time.perf_counter()
+    m = M.config3(nt=nst + 1, hmin_fn=lambda c, md: S.grid_ranges(512, 512, 256, c, md)["hmin"])
+    out = {}
+    for prec in (4, 8):
+        g = S.Solver(m, precision=prec, scheme=S.SCHEME_AWENO)
+        t1 = time.perf_counter()
+        done = 0
+        while done < nst:
+            n = min(250, nst - done)
+            g.step(n)
+            g.synchronize()
+            done += n
+            log(f"aweno accuracy: precision {prec}, {done}/{nst} steps")
+        dt = time.perf_counter() - t1
+        out[prec] = (g.stations(), g.pgv(), dt)
+        g.close()
+        log(f"aweno accuracy: precision {prec}, {nst} steps in {dt:.1f} s")
+    (s4, p4, t4), (s8, p8, t8) = out[4], out[8]
+    rel = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))
+    peak = np.abs(s8[:, :, :3]).max()
+    live = [q for q in range(len(m.stations)) if np.abs(s8[q, :, :3]).max() > 1e-3 * peak]
+    res = {"check": "ME-AWENO FP32 vs FP64", "workload": f"ME-AWENO basin {m.nx}x{m.ny}x{m.nz}, {len(m.stations)} "
+           f"stations, {nst} steps (dt {m.dt:.4e} s)", "station_velocity_rel_l2": rel(s4[:, :, :3], s8[:, :, :3]),
+           "stations_with_signal": len(live),
+           "worst_station_rel_l2": max(rel(s4[q, :, :3], s8[q, :, :3]) for q in live),
+           "pgv_rel_l2": rel(p4, p8), "fp32_ms_per_step": t4 * 1e3 / nst, "fp64_ms_per_step": t8 * 1e3 / nst,
+           "seconds": time.perf_counter() - t0}
+    print(json.dumps(res), flush=True)
 
 
 def cpu_leg(m, args):
@@ -471,7 +516,7 @@ def adjoint_fd_check(args):
 
     ts = [torch.tensor(m.medium[c], dtype=torch.float64, requires_grad=True) for c in range(3)]
     rate = torch.tensor(m.src_rate, dtype=torch.float64, requires_grad=True)
-    rec = A.seismograms(*ts, rate, m, nst, precision=8, checkpoint_every=max(1, nst // 10))
+    rec = A.seismograms(*ts, rate, m, nst, precision=8, checkpoint_every=args.checkpoint)
     loss = 0.5 * ((rec - torch.as_tensor(target)) ** 2).sum()
     loss.backward()
     grads = [t.grad.numpy() for t in ts] + [rate.grad.numpy()]
@@ -492,7 +537,8 @@ def adjoint_fd_check(args):
         errs.append(abs(fd - an) / abs(fd))
         log(f"direction {n}: finite difference {fd:.6e}, adjoint {an:.6e}, relative error {errs[-1]:.2e}")
     out = {"check": "adjoint gradient vs central differences", "workload": f"config 5 {m.nx}x{m.ny}x{m.nz}, "
-           f"{nst} steps, FP64, misfit against a 3% Vs-perturbed target", "directions": args.adjoint_fd,
+           f"{nst} steps, checkpoint every {args.checkpoint}, FP64, misfit against a 3% Vs-perturbed target",
+           "directions": args.adjoint_fd,
            "max_rel_error": max(errs), "median_rel_error": float(np.median(errs)),
            "seconds": time.perf_counter() - t0}
     print(json.dumps(out), flush=True)

@@ -578,14 +578,27 @@ __global__ __launch_bounds__(256) void cg_adj_s(AdjArgs<R> a, const TM* __restri
 // ---------------------------------------------------------------------------
 // small kernels of the adjoint driver
 // ---------------------------------------------------------------------------
-// adjoint source at the stations: mu[c](station) += src[s][n][c], c < 3; one
-// thread walks the list so stations sharing a cell add in order
+// adjoint source at the stations: mu[c](station) += src[s][n][c], c < 3;
+// stations sharing a cell add in station order
 template <class R>
 __global__ void cg_adj_inject(R* mu, long long N, int ns, const long long* __restrict__ cell, const R* __restrict__ src,
                               int nsteps, int n) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int s = 0; s < ns; ++s)
-    for (int c = 0; c < 3; ++c) mu[(long long)c * N + cell[s]] += src[((long long)s * nsteps + n) * 3 + c];
+  // one thread per cell: the first station of each cell adds the sources of
+  // all stations in that cell, in station order
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const long long p = cell[s];
+  for (int t = 0; t < s; ++t)
+    if (cell[t] == p) return;
+  R acc[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) acc[c] = mu[(long long)c * N + p];
+  for (int t = s; t < ns; ++t)
+    if (cell[t] == p)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[c] += src[((long long)t * nsteps + n) * 3 + c];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) mu[(long long)c * N + p] = acc[c];
 }
 
 // gradient of the scaled moment-rate table (time-major [t][p][6]) from the
@@ -664,7 +677,7 @@ int launch_adj_stage(AdjArgs<R> a, const TM* coef, const R* jac, const R* rdzdx,
 template <class R>
 void launch_adj_inject(R* mu, long long N, int ns, const long long* cell, const R* src, int nsteps, int n, hipStream_t st) {
   if (ns <= 0) return;
-  cg_adj_inject<R><<<1, 64, 0, st>>>(mu, N, ns, cell, src, nsteps, n);
+  cg_adj_inject<R><<<(ns + 63) / 64, 64, 0, st>>>(mu, N, ns, cell, src, nsteps, n);
 }
 
 template <class R>

@@ -488,10 +488,18 @@ class Kernel:
             w.sset(o[0], 1, g(o[1], 1))
         elif op == "s_mov_b64":
             w.sset(o[0], 2, g(o[1], 2))
-        elif op in ("s_and_b64", "s_or_b64", "s_andn2_b64", "s_and_b32", "s_or_b32"):
+        elif op in ("s_and_b64", "s_or_b64", "s_andn2_b64", "s_orn2_b64", "s_and_b32", "s_or_b32"):
             wd = 2 if op.endswith("64") else 1
             a, b = g(o[1], wd), g(o[2], wd)
-            r = a & b if "_and_" in op else (a | b if "_or_" in op else a & ~b & (M64 if wd == 2 else M32))
+            mask = M64 if wd == 2 else M32
+            if "_and_" in op:
+                r = a & b
+            elif "_or_" in op:
+                r = a | b
+            elif op.startswith("s_andn2"):
+                r = a & ~b & mask
+            else:
+                r = (a | ~b) & mask
             w.sset(o[0], wd, r)
             w.scc = int(r != 0)
         elif op in ("s_cselect_b64", "s_cselect_b32"):
