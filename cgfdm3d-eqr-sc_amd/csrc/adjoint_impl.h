@@ -83,26 +83,20 @@ __device__ __forceinline__ R g_of(const AdjArgs<R>& a, long long q) {
 // ---------------------------------------------------------------------------
 // pass A: per interior cell, products Z, PML auxiliary adjoint, bulk gradient
 // ---------------------------------------------------------------------------
-template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
-__global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restrict__ coef) {
-  CellTile t;
-  if (!interior_tile(blockIdx.x, a.nx, a.ny, a.nz, a.rtx, a.rty, t)) return;
-  const int i = H + t.i0 + (threadIdx.x & 63);
-  const int j = H + t.j0 + (threadIdx.x >> 6);
-  const int k = H + t.k;
-  if (i >= H + a.nx || j >= H + a.ny) return;
+// Pass A's work at interior cell (i, j, k) (padded index p) with the stage's
+// adjoint g of the cell in gv: products into Zout, PML auxiliary adjoint (the
+// next stage's g into fa[].gn when GN, else in place into fa[].g), gradient.
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE, bool GN>
+__device__ __forceinline__ void adj_a_cell(const AdjArgs<R>& a, const TM* __restrict__ coef, int i, int j, int k,
+                                           long long p, const R (&gv)[9], R* __restrict__ Zout) {
   const long long N = a.N, S = a.S;
-  const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
   auto cf = [&](int c) -> R { return R(Store<TM>::ld(coef + (long long)c * N + p)); };
   const R mu = cf(M_MU), lam = cf(M_LAM), bu = cf(M_B);
   const PmlArgs<R>& P = a.pml;
   const bool top = a.fs && k >= H + a.nz - 3;
-  R gv[9], gd[9];
+  R gd[9];
 #pragma unroll
-  for (int v = 0; v < 9; ++v) {
-    gv[v] = g_of<R, STAGE>(a, (long long)v * N + p);
-    gd[v] = gv[v] * a.DT;
-  }
+  for (int v = 0; v < 9; ++v) gd[v] = gv[v] * a.DT;
   double gmu = 0, glam = 0, gb = 0;
   const int FBv[3] = {FB1, FB2, FB3};
   const long long sv[3] = {1, (long long)a.NX, S};
@@ -179,7 +173,7 @@ __global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restri
             }
             const R lA = -ad * a.DT * gA - be * a.DT * gv[v];
             F.out[e] = base + lA;
-            if constexpr (STAGE > 0) F.g[e] = R(rk_b(STAGE - 1)) * fmu[v] + R(rk_a(STAGE)) * lA;
+            if constexpr (STAGE > 0) (GN ? F.gn : F.g)[e] = R(rk_b(STAGE - 1)) * fmu[v] + R(rk_a(STAGE)) * lA;
             gQ[v] = gA * a.DT;
           }
           R lD[9];
@@ -194,26 +188,38 @@ __global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restri
       }
     }
 #pragma unroll
-    for (int v = 0; v < 9; ++v) a.Z[(long long)(9 * ax + v) * N + p] = Zd[v];
+    for (int v = 0; v < 9; ++v) Zout[(long long)(9 * ax + v) * N + p] = Zd[v];
   }
   a.gmu[p] += gmu;
   a.glam[p] += glam;
   a.gb[p] += gb;
 }
 
-// ---------------------------------------------------------------------------
-// pass B: gather of the products, surface accumulator, RK combination
-// ---------------------------------------------------------------------------
-template <class R, int FB1, int FB2, int FB3, int STAGE>
-__global__ __launch_bounds__(256) void cg_adj_b(AdjArgs<R> a) {
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
+__global__ __launch_bounds__(256) void cg_adj_a(AdjArgs<R> a, const TM* __restrict__ coef) {
   CellTile t;
   if (!interior_tile(blockIdx.x, a.nx, a.ny, a.nz, a.rtx, a.rty, t)) return;
   const int i = H + t.i0 + (threadIdx.x & 63);
   const int j = H + t.j0 + (threadIdx.x >> 6);
   const int k = H + t.k;
   if (i >= H + a.nx || j >= H + a.ny) return;
+  const long long p = (long long)i + (long long)a.NX * j + (long long)k * a.S;
+  R gv[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) gv[v] = g_of<R, STAGE>(a, (long long)v * a.N + p);
+  adj_a_cell<R, TM, FB1, FB2, FB3, STAGE, false>(a, coef, i, j, k, p, gv, a.Z);
+}
+
+// ---------------------------------------------------------------------------
+// pass B: gather of the products, surface accumulator, RK combination
+// ---------------------------------------------------------------------------
+// Pass B's work at interior cell (i, j, k): gather of the products Zin,
+// surface accumulator, RK combination into out; the next stage's g of the
+// cell into gn (STAGE > 0).
+template <class R, int FB1, int FB2, int FB3, int STAGE>
+__device__ __forceinline__ void adj_b_cell(const AdjArgs<R>& a, const R* __restrict__ Zin, int k, long long p,
+                                           R (&gn)[9]) {
   const long long N = a.N, S = a.S;
-  const long long p = (long long)i + (long long)a.NX * j + (long long)k * S;
   const R cw[5] = {R(AF_1), R(AF0), R(AF1), R(AF2), R(AF3)};
   const int FBv[3] = {FB1, FB2, FB3};
   const long long sv[3] = {1, (long long)a.NX, S};
@@ -228,7 +234,7 @@ __global__ __launch_bounds__(256) void cg_adj_b(AdjArgs<R> a) {
     l[v] = R(0);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-      const R* z = a.Z + (long long)(9 * ax + v) * N + p;
+      const R* z = Zin + (long long)(9 * ax + v) * N + p;
       const long long d = FBv[ax] * sv[ax];
       R acc = R(0);
 #pragma unroll
@@ -247,8 +253,50 @@ __global__ __launch_bounds__(256) void cg_adj_b(AdjArgs<R> a) {
     if (in_ls) a.Ls[q] = R(0);
     if constexpr (STAGE == 3) a.out[q] = mv[v] + lv;
     else a.out[q] = ov[v] + lv;
-    if constexpr (STAGE > 0) a.G[q] = R(rk_b(STAGE - 1)) * mv[v] + R(rk_a(STAGE)) * lv;
+    if constexpr (STAGE > 0) gn[v] = R(rk_b(STAGE - 1)) * mv[v] + R(rk_a(STAGE)) * lv;
   }
+}
+
+template <class R, int FB1, int FB2, int FB3, int STAGE>
+__global__ __launch_bounds__(256) void cg_adj_b(AdjArgs<R> a) {
+  CellTile t;
+  if (!interior_tile(blockIdx.x, a.nx, a.ny, a.nz, a.rtx, a.rty, t)) return;
+  const int i = H + t.i0 + (threadIdx.x & 63);
+  const int j = H + t.j0 + (threadIdx.x >> 6);
+  const int k = H + t.k;
+  if (i >= H + a.nx || j >= H + a.ny) return;
+  const long long p = (long long)i + (long long)a.NX * j + (long long)k * a.S;
+  R gn[9];
+  adj_b_cell<R, FB1, FB2, FB3, STAGE>(a, a.Z, k, p, gn);
+  if constexpr (STAGE > 0) {
+#pragma unroll
+    for (int v = 0; v < 9; ++v) a.G[(long long)v * a.N + p] = gn[v];
+  }
+}
+
+// Pass B of stage STAGE (products in Z, F/B pattern FB) and pass A of stage
+// STAGE - 1 (products into Zn, the opposite F/B pattern) in one kernel: the
+// next stage's g of a cell goes from B to A in registers. It is stored only in
+// the three surface layers, where the surface pass of stage STAGE - 1 reads
+// it; that pass runs after this kernel and reads the auxiliary adjoint g of
+// stage STAGE - 1, so pass A writes the following one into fa[].gn.
+template <class R, class TM, int FB1, int FB2, int FB3, int STAGE>
+__global__ __launch_bounds__(256) void cg_adj_ba(AdjArgs<R> a, const TM* __restrict__ coef) {
+  static_assert(STAGE >= 1 && STAGE <= 3, "pass A follows pass B of stages 3..1");
+  CellTile t;
+  if (!interior_tile(blockIdx.x, a.nx, a.ny, a.nz, a.rtx, a.rty, t)) return;
+  const int i = H + t.i0 + (threadIdx.x & 63);
+  const int j = H + t.j0 + (threadIdx.x >> 6);
+  const int k = H + t.k;
+  if (i >= H + a.nx || j >= H + a.ny) return;
+  const long long p = (long long)i + (long long)a.NX * j + (long long)k * a.S;
+  R gn[9];
+  adj_b_cell<R, FB1, FB2, FB3, STAGE>(a, a.Z, k, p, gn);
+  if (a.fs && k >= H + a.nz - 3) {
+#pragma unroll
+    for (int v = 0; v < 9; ++v) a.G[(long long)v * a.N + p] = gn[v];
+  }
+  adj_a_cell<R, TM, -FB1, -FB2, -FB3, STAGE - 1, true>(a, coef, i, j, k, p, gn, a.Zn);
 }
 
 template <class R>
@@ -631,26 +679,31 @@ __global__ void cg_record_vel(const R* __restrict__ W, long long N, int ns, cons
 // the three passes of one adjoint stage for one F/B pattern
 template <class R, class TM, int F1, int F2, int F3, int STAGE>
 void adj_passes(const AdjArgs<R>& a, const TM* coef, const R* jac, const R* rdzdx, const R* rdzdy, dim3 grid, dim3 gs,
-                dim3 block, hipStream_t st) {
-  if (a.fs) cg_adj_s<R, TM, F1, F2, F3, STAGE><<<gs, block, 0, st>>>(a, coef, jac, rdzdx, rdzdy);
-  cg_adj_a<R, TM, F1, F2, F3, STAGE><<<grid, block, 0, st>>>(a, coef);
-  cg_adj_b<R, F1, F2, F3, STAGE><<<grid, block, 0, st>>>(a);
+                dim3 block, int mode, hipStream_t st) {
+  const bool s = mode == ADJ_FULL || mode == ADJ_SA || mode == ADJ_S;
+  if (a.fs && s) cg_adj_s<R, TM, F1, F2, F3, STAGE><<<gs, block, 0, st>>>(a, coef, jac, rdzdx, rdzdy);
+  if (mode == ADJ_FULL || mode == ADJ_SA) cg_adj_a<R, TM, F1, F2, F3, STAGE><<<grid, block, 0, st>>>(a, coef);
+  if (mode == ADJ_FULL || mode == ADJ_B) cg_adj_b<R, F1, F2, F3, STAGE><<<grid, block, 0, st>>>(a);
+  if constexpr (STAGE > 0)
+    if (mode == ADJ_BA) cg_adj_ba<R, TM, F1, F2, F3, STAGE><<<grid, block, 0, st>>>(a, coef);
 }
 
 template <class R, class TM, int F1, int F2, int F3>
 void adj_stage_fb(const AdjArgs<R>& a, const TM* coef, const R* jac, const R* rdzdx, const R* rdzdy, dim3 grid, dim3 gs,
-                  dim3 block, hipStream_t st) {
+                  dim3 block, int mode, hipStream_t st) {
   switch (a.stage) {
-    case 0: adj_passes<R, TM, F1, F2, F3, 0>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
-    case 1: adj_passes<R, TM, F1, F2, F3, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
-    case 2: adj_passes<R, TM, F1, F2, F3, 2>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
-    default: adj_passes<R, TM, F1, F2, F3, 3>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    case 0: adj_passes<R, TM, F1, F2, F3, 0>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
+    case 1: adj_passes<R, TM, F1, F2, F3, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
+    case 2: adj_passes<R, TM, F1, F2, F3, 2>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
+    default: adj_passes<R, TM, F1, F2, F3, 3>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
   }
 }
 
 template <class R, class TM>
 int launch_adj_stage(AdjArgs<R> a, const TM* coef, const R* jac, const R* rdzdx, const R* rdzdy, int fb1, int fb2,
-                     int fb3, hipStream_t st) {
+                     int fb3, int mode, hipStream_t st) {
+  if (mode < ADJ_FULL || mode > ADJ_B) return LAUNCH_BAD_STAGE;
+  if (mode == ADJ_BA && a.stage < 1) return LAUNCH_BAD_STAGE;
   if (a.stage < 0 || a.stage > 3) return LAUNCH_BAD_STAGE;
   if (a.nx <= 0 || a.ny <= 0 || a.nz <= 0) return LAUNCH_EMPTY;
   if ((a.ny + 3) / 4 > 65535) return LAUNCH_TOO_LARGE;
@@ -662,14 +715,14 @@ int launch_adj_stage(AdjArgs<R> a, const TM* coef, const R* jac, const R* rdzdx,
   // the surface pass reads the auxiliary adjoint g of this stage, which pass A
   // then overwrites with that of the next stage: surface, A, B in this order
   switch (fb_index(fb1, fb2, fb3)) {
-    case 0: adj_stage_fb<R, TM, -1, -1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
-    case 1: adj_stage_fb<R, TM, 1, -1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
-    case 2: adj_stage_fb<R, TM, -1, 1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
-    case 3: adj_stage_fb<R, TM, 1, 1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
-    case 4: adj_stage_fb<R, TM, -1, -1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
-    case 5: adj_stage_fb<R, TM, 1, -1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
-    case 6: adj_stage_fb<R, TM, -1, 1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
-    default: adj_stage_fb<R, TM, 1, 1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, st); break;
+    case 0: adj_stage_fb<R, TM, -1, -1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
+    case 1: adj_stage_fb<R, TM, 1, -1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
+    case 2: adj_stage_fb<R, TM, -1, 1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
+    case 3: adj_stage_fb<R, TM, 1, 1, -1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
+    case 4: adj_stage_fb<R, TM, -1, -1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
+    case 5: adj_stage_fb<R, TM, 1, -1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
+    case 6: adj_stage_fb<R, TM, -1, 1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
+    default: adj_stage_fb<R, TM, 1, 1, 1>(a, coef, jac, rdzdx, rdzdy, grid, gs, block, mode, st); break;
   }
   return LAUNCH_OK;
 }

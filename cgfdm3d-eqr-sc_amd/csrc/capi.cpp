@@ -389,6 +389,10 @@ struct Solver : SolverBase {
   // planes. CGFDM_INTERIOR_KC (-1..4096) overrides it.
   int kc = 0;
   int zty = 8;  // z-marching tile height (CGFDM_INTERIOR_ZTY: 4 or 8)
+  // adjoint passes: 1 pass B of each stage fused with pass A of the next
+  // (the stage's g passed in registers), 0 the three passes per stage;
+  // CGFDM_ADJOINT_PIPE overrides it
+  int adj_pipe = 1;
   // register z-marching kernel: planes per chunk (0 = off); CGFDM_INTERIOR_ZREG
   // (0..4096) overrides it
   int zreg = 0;
@@ -408,6 +412,7 @@ struct Solver : SolverBase {
       if (v >= -1 && v <= 4096) kc = v;
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
+    if (const char* e = std::getenv("CGFDM_ADJOINT_PIPE")) adj_pipe = std::atoi(e) != 0 ? 1 : 0;
     if (const char* e = std::getenv("CGFDM_INTERIOR_ZREG")) {
       const int v = std::atoi(e);
       if (v >= 0 && v <= 4096) zreg = v;
@@ -921,7 +926,7 @@ struct Solver : SolverBase {
     long long fcnt = 0;
     for (int f = 0; f < 6; ++f) fcnt += 9 * pml.f[f].cnt;
     const double r = (double)sizeof(R);
-    return r * (72.0 * N + 3.0 * std::max(fcnt, 1LL) + (double)state_size() * (ck + 1) +
+    return r * (99.0 * N + 4.0 * std::max(fcnt, 1LL) + (double)state_size() * (ck + 1) +
                 3.0 * std::max(ns, 1) * nsteps) +
            8.0 * (3.0 * N + 6.0 * std::max((long long)src_np * src_nt, 1LL));
   }
@@ -1077,8 +1082,8 @@ struct Solver : SolverBase {
     long long fcnt = 0;
     for (int f = 0; f < 6; ++f) fcnt += 9 * pml.f[f].cnt;
     if (!adj_buf) {
-      if ((rc = dalloc(&adj_buf, 72 * N))) return rc;
-      if ((rc = dalloc(&adj_face, std::max(3 * fcnt, 1LL)))) return rc;
+      if ((rc = dalloc(&adj_buf, 99 * N))) return rc;
+      if ((rc = dalloc(&adj_face, std::max(4 * fcnt, 1LL)))) return rc;
       if ((rc = dalloc(&adj_grad, 3 * N))) return rc;
       if ((rc = dalloc(&adj_gsrc, std::max((long long)src_np * src_nt * 6, 1LL)))) return rc;
       if ((rc = dalloc(&adj_seg, ss * fr_ck))) return rc;
@@ -1091,8 +1096,9 @@ struct Solver : SolverBase {
     R* Z = adj_buf + 27 * N;
     R* Ls = adj_buf + 54 * N;
     R* X3 = adj_buf + 63 * N;
-    HIPCHK(hipMemsetAsync(adj_buf, 0, sizeof(R) * 72 * N, stream));
-    HIPCHK(hipMemsetAsync(adj_face, 0, sizeof(R) * std::max(3 * fcnt, 1LL), stream));
+    R* Z2 = adj_buf + 72 * N;  // the fused pass B + A writes the next stage's products here
+    HIPCHK(hipMemsetAsync(adj_buf, 0, sizeof(R) * 99 * N, stream));
+    HIPCHK(hipMemsetAsync(adj_face, 0, sizeof(R) * std::max(4 * fcnt, 1LL), stream));
     HIPCHK(hipMemsetAsync(adj_grad, 0, sizeof(double) * 3 * N, stream));
     HIPCHK(hipMemsetAsync(adj_gsrc, 0, sizeof(double) * std::max((long long)src_np * src_nt * 6, 1LL), stream));
     if (ns > 0) {
@@ -1106,8 +1112,9 @@ struct Solver : SolverBase {
       long long off = 0;
       for (int f = 0; f < 6; ++f) {
         const long long c = 9 * pml.f[f].cnt;
-        fa[f] = cg::FaceAdj<R>{adj_face + off, adj_face + off + c, adj_face + off + 2 * c, pml.f[f].cnt};
-        off += 3 * c;
+        fa[f] = cg::FaceAdj<R>{adj_face + off, adj_face + off + c, adj_face + off + 2 * c, pml.f[f].cnt,
+                               adj_face + off + 3 * c};
+        off += 4 * c;
       }
     }
     alloc_stage_store(fr_ck, 0.0);
@@ -1147,7 +1154,9 @@ struct Solver : SolverBase {
           X[0] = A; X[1] = B; X[2] = C; X[3] = X3;
         }
         const int* fb0 = FBTAB[step % 8];
-        for (int s = 3; s >= 0; --s) {
+        // one launch of the transposed stage passes: stage s, F/B pattern of
+        // stage s, stage state x, products read (zr) and written (zw)
+        auto passes = [&](int mode, int s, const R* x, R* zr, R* zw) -> int {
           const int sg = (s % 2 == 0) ? 1 : -1;
           cg::AdjArgs<R> a{};
           a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.NX = NX; a.NY = NY; a.N = N; a.S = S;
@@ -1156,13 +1165,33 @@ struct Solver : SolverBase {
           a.stage = s;
           a.DT = R(d.dt);
           a.rDH = R(1.0 / d.dh);
-          a.X = X[s];
-          a.mu = MU; a.out = OUT; a.G = G; a.Z = Z; a.Ls = Ls;
+          a.X = x;
+          a.mu = MU; a.out = OUT; a.G = G; a.Z = zr; a.Zn = zw; a.Ls = Ls;
           a.gmu = adj_grad; a.glam = adj_grad + N; a.gb = adj_grad + 2 * N;
           a.pml = pml;
           for (int f = 0; f < 6; ++f) a.fa[f] = fa[f];
           if constexpr (std::is_same<R, TM>::value)
-            LAUNCHCHK((cg::launch_adj_stage<R, TM>(a, coef, jac, rdzdx, rdzdy, sg * fb0[0], sg * fb0[1], sg * fb0[2], stream)), "adjoint stage");
+            LAUNCHCHK((cg::launch_adj_stage<R, TM>(a, coef, jac, rdzdx, rdzdy, sg * fb0[0], sg * fb0[1], sg * fb0[2],
+                                                   mode, stream)), "adjoint stage");
+          return CGFDM_OK;
+        };
+        if (adj_pipe) {
+          // surface + A of stage 3, then B of stage s with A of stage s - 1
+          // (the surface pass of stage s - 1 after it), then B of stage 0
+          R* zc = Z;
+          R* zo = Z2;
+          if ((rc = passes(cg::ADJ_SA, 3, X[3], zc, zo))) return rc;
+          for (int s = 3; s >= 1; --s) {
+            if ((rc = passes(cg::ADJ_BA, s, X[s - 1], zc, zo))) return rc;
+            std::swap(zc, zo);
+            if (d.free_surface && (rc = passes(cg::ADJ_S, s - 1, X[s - 1], zc, zo))) return rc;
+            if (s - 1 > 0)
+              for (int f = 0; f < 6; ++f) std::swap(fa[f].g, fa[f].gn);
+          }
+          if ((rc = passes(cg::ADJ_B, 0, X[0], zc, zo))) return rc;
+        } else {
+          for (int s = 3; s >= 0; --s)
+            if ((rc = passes(cg::ADJ_FULL, s, X[s], Z, Z2))) return rc;
         }
         if (srcIt >= 0)
           cg::launch_adj_source<R>(OUT, N, src_np, src_cell, adj_gsrc, srcIt, (double)tw, (double)R(d.dt), stream);
