@@ -20,19 +20,27 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 pytestmark = pytest.mark.skipif(os.environ.get("CGFDM_ISA_TESTS") != "1",
                                 reason="slow (minutes per case): set CGFDM_ISA_TESTS=1")
 
-# (scheme, case, precision, steps, extra, tolerance vs the oracle)
+# (scheme, case, precision, steps, extra, tolerance vs the oracle, environment);
+# extra: coefficient storage (cgfdm) or checkpoint interval (adjoint)
 CASES = [
-    ("cgfdm", "curvilinear_pml_fs", 8, 2, 0, 1e-12),
-    ("cgfdm", "no_free_surface", 4, 2, 0, 1e-5),
-    ("cgfdm", "curvilinear_pml_fs", 4, 1, 1, 5e-3),
-    ("aweno", "aweno_curvilinear_fs", 8, 1, 0, 1e-12),
+    ("cgfdm", "curvilinear_pml_fs", 8, 2, 0, 1e-12, {}),
+    ("cgfdm", "no_free_surface", 4, 2, 0, 1e-5, {}),
+    ("cgfdm", "curvilinear_pml_fs", 4, 1, 1, 5e-3, {}),
+    ("aweno", "aweno_curvilinear_fs", 8, 1, 0, 1e-12, {}),
+    ("aweno", "aweno_two_x_tiles", 4, 1, 0, 1e-5, {}),
+    # pipelined adjoint passes over two checkpoint segments, with the stage store
+    # (the stand-in runtime reports no free memory, so the budget is given) and
+    # without it (stages recomputed per step)
+    ("adjoint", "pml_fs_curvilinear", 8, 3, 2, 1e-12, {"CGFDM_ADJOINT_STORE_GB": "1"}),
+    ("adjoint", "pml_no_fs", 8, 2, 5, 1e-12, {}),
 ]
 
 
-@pytest.mark.parametrize("scheme,name,prec,steps,extra,tol", CASES)
-def test_compiled_kernels_run_in_bounds_and_match_the_oracle(scheme, name, prec, steps, extra, tol):
+@pytest.mark.parametrize("scheme,name,prec,steps,extra,tol,env", CASES)
+def test_compiled_kernels_run_in_bounds_and_match_the_oracle(scheme, name, prec, steps, extra, tol, env):
     r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "isa", "run_case.py"), scheme, name, str(prec),
-                        str(steps), str(extra)], capture_output=True, text=True, timeout=3000)
+                        str(steps), str(extra)], capture_output=True, text=True, timeout=3000,
+                       env=dict(os.environ, **env))
     line = [l for l in r.stdout.splitlines() if l.startswith("RESULT")]
     assert r.returncode == 0 and line, r.stdout[-2000:] + r.stderr[-2000:]
     assert "fault None" in line[0], line[0]
