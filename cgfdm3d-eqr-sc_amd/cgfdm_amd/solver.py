@@ -251,7 +251,8 @@ class Solver:
         set_source ([npts, nt, 6])."""
         NZ, NY, NX = self.shape
         src = np.ascontiguousarray(src, np.float64)
-        gmu, glam, gb = (np.zeros(self.N) for _ in range(3))
+        # every element is written by the library
+        gmu, glam, gb = (np.empty(self.N) for _ in range(3))
         npts = 0 if self.m.src_cells is None else len(self.m.src_cells)
         nt = 0 if self.m.src_rate is None else self.m.src_rate.shape[1]
         grate = np.zeros((max(npts, 1), max(nt, 1), 6))

@@ -972,22 +972,31 @@ struct Solver : SolverBase {
   }
   // one forward step (no observers) that leaves its stage states in x[0..3]:
   // x0 = W after the source, x1..x3 = inputs of stages 1..3
-  int step_store(R* x0, R* x1, R* x2, R* x3, bool observe = false) {
+  // One forward step whose stage states stay in the 36N block xs: x0 = the
+  // state after the source (the step runs on it in place of A), x1..x3 = the
+  // inputs of stages 1..3. The step's result goes to `next` (the x0 of the next
+  // step's block) or, when next is null, to A; `first` copies A into x0 first.
+  // A chain of steps thus moves the state from block to block without copies.
+  int step_store(R* xs, R* next, bool first, bool observe = false) {
+    R* x0 = xs;
+    R* x1 = xs + 9 * N;
+    R* x2 = xs + 18 * N;
+    R* x3 = xs + 27 * N;
     const int kend = cg::H + d.nz - (d.free_surface ? 3 : 0);
     int fb[3] = {FBTAB[it % 8][0], FBTAB[it % 8][1], FBTAB[it % 8][2]};
     float tw = 0.f;
     const int srcIt = source_index(it, tw);
-    if (srcIt >= 0) cg::launch_source<R>(A, N, src_np, src_cell, src_rate, srcIt, R(tw), R(d.dt), stream);
-    HIPCHK(hipMemcpyAsync(x0, A, sizeof(R) * 9 * N, hipMemcpyDeviceToDevice, stream));
-    R* ins[4] = {A, x1, x2, x3};
-    R* outs[4] = {x1, x2, x3, A};
+    if (first) HIPCHK(hipMemcpyAsync(x0, A, sizeof(R) * 9 * N, hipMemcpyDeviceToDevice, stream));
+    if (srcIt >= 0) cg::launch_source<R>(x0, N, src_np, src_cell, src_rate, srcIt, R(tw), R(d.dt), stream);
+    R* ins[4] = {x0, x1, x2, x3};
+    R* outs[4] = {x1, x2, x3, next ? next : A};
     for (int irk = 0; irk < 4; ++irk) {
       cg::StageArgs<R> a{};
       a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.NX = NX; a.NY = NY; a.N = N; a.S = S;
       a.rowb = (unsigned)(NX * sizeof(R)); a.planeb = (unsigned)(S * sizeof(R));
       a.kbeg = cg::H; a.kend = kend; a.stage = irk; a.rty = rty; a.rtx = rtx; a.zreg = zreg; a.kc = kc; a.zty = zty;
       a.DT = R(d.dt); a.rDH = R(1.0 / d.dh); a.aff = aff; a.nzp = NZ;
-      a.win = ins[irk]; a.wout = outs[irk]; a.wm = A; a.wt = T; a.pml = pml;
+      a.win = ins[irk]; a.wout = outs[irk]; a.wm = x0; a.wt = T; a.pml = pml;
       LAUNCHCHK((cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream)), "interior stage");
       if (d.free_surface) {
         cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, observe ? pgv : nullptr, dis};
@@ -1053,13 +1062,16 @@ struct Solver : SolverBase {
     const int last0 = (nck - 1) * ck;
     for (int n = 0; n < nsteps; ++n) {
       if (n % ck == 0 && (rc = copy_state(ck_store + (long long)(n / ck) * ss, true))) return rc;
+      const R* now = A;  // the state after this step
       if (adj_stages && n >= last0) {
         R* xs = adj_stages + 36LL * N * (n - last0);
-        if ((rc = step_store(xs, xs + 9 * N, xs + 18 * N, xs + 27 * N, true))) return rc;
+        R* next = n + 1 < nsteps ? xs + 36LL * N : nullptr;
+        if ((rc = step_store(xs, next, n == last0, true))) return rc;
+        if (next) now = next;
       } else if ((rc = step_impl(1, true))) {
         return rc;
       }
-      cg::launch_record_vel<R>(A, N, ns, st_cell, rec_vel, nsteps, n, stream);
+      cg::launch_record_vel<R>(now, N, ns, st_cell, rec_vel, nsteps, n, stream);
     }
     if (adj_stages) stages_seg = nck - 1;
     HIPCHK(hipGetLastError());
@@ -1129,7 +1141,7 @@ struct Solver : SolverBase {
         for (int n = n0; n < n1; ++n) {
           if (adj_stages) {
             R* xs = adj_stages + 36LL * N * (n - n0);
-            if ((rc = step_store(xs, xs + 9 * N, xs + 18 * N, xs + 27 * N))) return rc;
+            if ((rc = step_store(xs, n + 1 < n1 ? xs + 36LL * N : nullptr, n == n0))) return rc;
           } else {
             if ((rc = copy_state(adj_seg + (long long)(n - n0) * ss, true))) return rc;
             if ((rc = step_impl(1, false))) return rc;
@@ -1202,29 +1214,36 @@ struct Solver : SolverBase {
     HIPCHK(hipGetLastError());
     if ((rc = copy_state(adj_end, false))) return rc;
     it = fr_it0 + fr_n;
-    std::vector<double> hg(3 * (size_t)N);
-    HIPCHK(hipMemcpyAsync(hg.data(), adj_grad, sizeof(double) * hg.size(), hipMemcpyDeviceToHost, stream));
+    // gradients to the tight layout of the C ABI on the device (strided
+    // device copies into the work buffer, no longer needed), then one
+    // contiguous copy into each of the caller's arrays
+    double* outs[3] = {gmu, glam, gb};
+    double* tight = reinterpret_cast<double*>(adj_buf);  // 3 tN doubles fit in its 99N elements of R
+    for (int g = 0; g < 3; ++g)
+      if (outs[g]) {
+        HIPCHK(hipMemcpy2DAsync(tight + (size_t)g * tN, sizeof(double) * tNX, adj_grad + (size_t)g * N,
+                                sizeof(double) * NX, sizeof(double) * tNX, (size_t)NY * NZ, hipMemcpyDeviceToDevice,
+                                stream));
+        HIPCHK(hipMemcpyAsync(outs[g], tight + (size_t)g * tN, sizeof(double) * tN, hipMemcpyDeviceToHost, stream));
+      }
     std::vector<double> hs((size_t)src_np * src_nt * 6);
     if (src_np > 0)
       HIPCHK(hipMemcpyAsync(hs.data(), adj_gsrc, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    // scaled rate = raw mu(cell) / (J DH^3): the scaling carries mu of the source cell
+    // scaled rate = raw mu(cell) / (J DH^3): the scaling carries mu of the source
+    // cell (device index -> tight index of the same cell)
     std::vector<long long> hcell(src_np);
     if (src_np > 0) HIPCHK(hipMemcpy(hcell.data(), src_cell, sizeof(long long) * src_np, hipMemcpyDeviceToHost));
-    for (int p = 0; p < src_np; ++p)
-      for (int t = 0; t < src_nt; ++t)
-        for (int q = 0; q < 6; ++q) {
-          const size_t e = ((size_t)t * src_np + p) * 6 + q;
-          hg[hcell[p]] += hs[e] * src_raw[e] * src_vol[p];
-        }
-    // device layout -> tight layout of the C ABI
-    double* outs[3] = {gmu, glam, gb};
-    for (int g = 0; g < 3; ++g) {
-      if (!outs[g]) continue;
-      const double* src = hg.data() + (size_t)g * N;
-      for (long long r = 0; r < (long long)NY * NZ; ++r)
-        std::memcpy(outs[g] + r * tNX, src + r * NX, sizeof(double) * tNX);
-    }
+    if (gmu)
+      for (int p = 0; p < src_np; ++p) {
+        const long long c = hcell[p], ci = c % NX, cr = c / NX;  // cr = j + NY k
+        const long long tc = ci + (long long)tNX * cr;
+        for (int t = 0; t < src_nt; ++t)
+          for (int q = 0; q < 6; ++q) {
+            const size_t e = ((size_t)t * src_np + p) * 6 + q;
+            gmu[tc] += hs[e] * src_raw[e] * src_vol[p];
+          }
+      }
     if (grate)
       for (size_t pi = 0; pi < src_of_point.size(); ++pi) {
         const int p = src_of_point[pi];
