@@ -74,10 +74,11 @@ constexpr double REF_PI = 3.141592657;  // reference macro.h:70, enters the PML 
 struct SolverBase {
   virtual ~SolverBase() {
     for (auto& e : ev) {
-      hipEventDestroy(e.a);
-      hipEventDestroy(e.b);
+      (void)hipEventDestroy(e.a);
+      (void)hipEventDestroy(e.b);
     }
-    for (void* p : allocs) hipFree(p);
+    // teardown: nothing to report to
+    for (void* p : allocs) (void)hipFree(p);
   }
   virtual int init(const cgfdm_desc& d, const float* coord, const float* medium) = 0;
   virtual int set_source(int np, int nt, double dt, const long long* cell, const float* rate) = 0;
@@ -217,7 +218,7 @@ struct SolverBase {
   void dfree(void* p) {
     for (size_t q = 0; q < allocs.size(); ++q)
       if (allocs[q] == p) {
-        hipFree(p);
+        (void)hipFree(p);  // a failed free leaves nothing to recover
         allocs.erase(allocs.begin() + q);
         return;
       }
@@ -235,8 +236,9 @@ struct SolverBase {
   size_t next_pair(int kind) {
     if (ev_used >= ev.size()) {
       EvPair e{};
-      hipEventCreate(&e.a);
-      hipEventCreate(&e.b);
+      // a failed creation surfaces at the checked hipEventRecord of this pair
+      (void)hipEventCreate(&e.a);
+      (void)hipEventCreate(&e.b);
       ev.push_back(e);
     }
     ev[ev_used].kind = kind;
@@ -1634,9 +1636,9 @@ int cgfdm_grid_ranges(int nx, int ny, int nz, const float* coord, const float* m
   float *dc = nullptr, *dm = nullptr;
   double* part = nullptr;
   auto cleanup = [&]() {
-    if (dc) hipFree(dc);
-    if (dm) hipFree(dm);
-    if (part) hipFree(part);
+    if (dc) (void)hipFree(dc);
+    if (dm) (void)hipFree(dm);
+    if (part) (void)hipFree(part);
   };
   if (hipMalloc(&dc, sizeof(float) * 3 * N) != hipSuccess || hipMalloc(&dm, sizeof(float) * 3 * N) != hipSuccess ||
       hipMalloc(&part, sizeof(double) * 8 * nb) != hipSuccess) {
@@ -1688,7 +1690,7 @@ const char* cgfdm_kernel_name(cgfdm_solver* s, int which) {
 int cgfdm_destroy(cgfdm_solver* s) {
   if (!s) return CGFDM_OK;
   cgfdm_detail::release_session(s);
-  if (s->impl && s->impl->stream) hipStreamSynchronize(s->impl->stream);
+  if (s->impl && s->impl->stream) (void)hipStreamSynchronize(s->impl->stream);  // drain before the frees
   delete s;
   return CGFDM_OK;
 }
