@@ -323,6 +323,18 @@ float bits16_to_float(unsigned short) { return 0.f; }
 template <class R, class TM>
 struct Solver : SolverBase {
   cgfdm_desc d{};
+  // surface stage kernels on a second stream, forked from and joined to the
+  // solver's stream around every stage (CGFDM_SURFACE_STREAM=0: one stream):
+  // they write only the top three layers, which the interior kernel of the
+  // same stage neither writes nor reads, so the two run side by side
+  int surf_overlap = 1;
+  hipStream_t sstream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  ~Solver() override {
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (sstream) (void)hipStreamDestroy(sstream);
+  }
   // device layout: row pitch NX (nx + 6 rounded up to 32 elements in the
   // row-aligned layout), plane S = NX NY, volume N = S NZ; the host-facing
   // arrays of the C ABI use the tight layout tNX = nx + 6 (tS, tN)
@@ -415,6 +427,7 @@ struct Solver : SolverBase {
     }
     if (const char* e = std::getenv("CGFDM_INTERIOR_ZTY")) zty = std::atoi(e) == 4 ? 4 : 8;
     if (const char* e = std::getenv("CGFDM_ADJOINT_PIPE")) adj_pipe = std::atoi(e) != 0 ? 1 : 0;
+    if (const char* e = std::getenv("CGFDM_SURFACE_STREAM")) surf_overlap = std::atoi(e) != 0 ? 1 : 0;
     if (const char* e = std::getenv("CGFDM_INTERIOR_ZREG")) {
       const int v = std::atoi(e);
       if (v >= 0 && v <= 4096) zreg = v;
@@ -705,6 +718,12 @@ struct Solver : SolverBase {
   int step_impl(int n, bool observe) {
     if (!kCgfdmStepEnabled) return fail(CGFDM_ERR_DISABLED, kCgfdmStepOff);
     const int kend = cg::H + d.nz - (d.free_surface ? 3 : 0);
+    const bool fork = d.free_surface && surf_overlap;
+    if (fork && !sstream) {
+      HIPCHK(hipStreamCreateWithFlags(&sstream, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
     for (int s = 0; s < n; ++s) {
       int fb[3] = {FBTAB[it % 8][0], FBTAB[it % 8][1], FBTAB[it % 8][2]};
       size_t sp = 0;
@@ -752,6 +771,24 @@ struct Solver : SolverBase {
         a.wm = A;
         a.wt = T;
         a.pml = pml;
+        // the surface stage (forked first, so that its few blocks are
+        // dispatched before the interior's) and the interior stage
+        if (d.free_surface) {
+          hipStream_t ss = fork ? sstream : stream;
+          if (fork) {
+            HIPCHK(hipEventRecord(ev_fork, stream));
+            HIPCHK(hipStreamWaitEvent(sstream, ev_fork, 0));
+          }
+          cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, observe ? pgv : nullptr, dis};
+          size_t fp = 0;
+          if (timing_on) {
+            fp = next_pair(1);
+            HIPCHK(hipEventRecord(ev[fp].a, ss));
+          }
+          LAUNCHCHK((cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], ss)), "surface stage");
+          if (timing_on) HIPCHK(hipEventRecord(ev[fp].b, ss));
+          if (fork) HIPCHK(hipEventRecord(ev_join, sstream));
+        }
         size_t ip = 0;
         if (timing_on) {
           ip = next_pair(0);
@@ -759,16 +796,7 @@ struct Solver : SolverBase {
         }
         LAUNCHCHK((cg::launch_interior<R, TM>(a, coef, fb[0], fb[1], fb[2], stream)), "interior stage");
         if (timing_on) HIPCHK(hipEventRecord(ev[ip].b, stream));
-        if (d.free_surface) {
-          cg::SurfArgs<R> sa{jac, rdzdx, rdzdy, observe ? pgv : nullptr, dis};
-          size_t fp = 0;
-          if (timing_on) {
-            fp = next_pair(1);
-            HIPCHK(hipEventRecord(ev[fp].a, stream));
-          }
-          LAUNCHCHK((cg::launch_surface<R, TM>(a, coef, sa, fb[0], fb[1], fb[2], stream)), "surface stage");
-          if (timing_on) HIPCHK(hipEventRecord(ev[fp].b, stream));
-        }
+        if (fork) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
         fb[0] = -fb[0];
         fb[1] = -fb[1];
         fb[2] = -fb[2];

@@ -94,6 +94,11 @@ hipError_t hipGetSymbolAddress(void** p, const void* sym) { (void)sym; *p = 0; r
 /* no free-memory figure: optional stores (the adjoint's stage states) are not taken */
 hipError_t hipMemGetInfo(size_t* fre, size_t* tot) { *fre = 0; *tot = 0; return 0; }
 hipError_t hipEventCreate(hipEvent_t* e) { *e = (void*)1; return 0; }
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags) { (void)flags; *e = (void*)1; return 0; }
+/* launches run to completion in order, so a second stream is the first one */
+hipError_t hipStreamCreateWithFlags(hipStream_t* st, unsigned flags) { (void)flags; *st = (void*)2; return 0; }
+hipError_t hipStreamDestroy(hipStream_t st) { (void)st; return 0; }
+hipError_t hipStreamWaitEvent(hipStream_t st, hipEvent_t e, unsigned flags) { (void)st; (void)e; (void)flags; return 0; }
 hipError_t hipEventDestroy(hipEvent_t e) { (void)e; return 0; }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t st) { (void)e; (void)st; return 0; }
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) { (void)a; (void)b; *ms = 0.f; return 0; }
