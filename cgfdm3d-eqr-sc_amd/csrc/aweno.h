@@ -28,7 +28,7 @@ struct StageArgs {
 template <class R>
 void launch_faces(const StageArgs<R>& a, hipStream_t st);
 template <class R>
-void launch_update(const StageArgs<R>& a, int nx, int ny, hipStream_t st);
+void launch_update(const StageArgs<R>& a, int nx, int ny, hipStream_t st, hipStream_t st_top);
 template <class R>
 void launch_char_surface(R* W, const R* cjm, int nx, int ny, int NX, int NZ, long long N, long long S, hipStream_t st);
 template <class R>

@@ -3,7 +3,7 @@
 
 namespace aw {
 template void launch_faces<float>(const StageArgs<float>&, hipStream_t);
-template void launch_update<float>(const StageArgs<float>&, int, int, hipStream_t);
+template void launch_update<float>(const StageArgs<float>&, int, int, hipStream_t, hipStream_t);
 template void launch_char_surface<float>(float*, const float*, int, int, int, int, long long, long long, hipStream_t);
 template void launch_sponge<float>(float*, int, int, int, int, long long, long long, int, hipStream_t);
 template void launch_source<float>(float*, const float*, long long, int, const long long*, const float*, int, float,

@@ -3,7 +3,7 @@
 
 namespace aw {
 template void launch_faces<double>(const StageArgs<double>&, hipStream_t);
-template void launch_update<double>(const StageArgs<double>&, int, int, hipStream_t);
+template void launch_update<double>(const StageArgs<double>&, int, int, hipStream_t, hipStream_t);
 template void launch_char_surface<double>(double*, const double*, int, int, int, int, long long, long long,
                                           hipStream_t);
 template void launch_sponge<double>(double*, int, int, int, int, long long, long long, int, hipStream_t);

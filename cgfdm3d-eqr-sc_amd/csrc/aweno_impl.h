@@ -592,12 +592,14 @@ void launch_faces(const StageArgs<R>& a, hipStream_t st) {
 }
 
 template <class R>
-void launch_update(const StageArgs<R>& a, int nx, int ny, hipStream_t st) {
+void launch_update(const StageArgs<R>& a, int nx, int ny, hipStream_t st, hipStream_t st_top) {
+  // the top three layers (their own kernel, launched first) and the rest
+  // write disjoint cells and read only the stage input and the face fluxes
   const dim3 block(256);
+  const dim3 gtop((nx + 63) / 64, (ny + 3) / 4, 3);
+  aw_update_top<R><<<gtop, block, 0, st_top>>>(a, nx, ny);
   const long long nb = cg::interior_blocks(a.XE - 2 * H, a.NY, a.NZ, a.rtx, a.rty);
   aw_update<R><<<dim3((unsigned)nb), block, 0, st>>>(a);
-  const dim3 gtop((nx + 63) / 64, (ny + 3) / 4, 3);
-  aw_update_top<R><<<gtop, block, 0, st>>>(a, nx, ny);
 }
 
 template <class R>

@@ -1293,6 +1293,16 @@ struct Solver : SolverBase {
 template <class R>
 struct AwenoSolver : SolverBase {
   cgfdm_desc d{};
+  // the top-layer update on a second stream beside the bulk update, forked
+  // from and joined to the solver's stream (CGFDM_SURFACE_STREAM=0: one stream)
+  int surf_overlap = 1;
+  hipStream_t sstream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  ~AwenoSolver() override {
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (sstream) (void)hipStreamDestroy(sstream);
+  }
   // device layout (row pitch NX) and the tight layout of the C ABI (tNX), as
   // in the CGFDM solver
   int NX = 0, NY = 0, NZ = 0;
@@ -1331,6 +1341,7 @@ struct AwenoSolver : SolverBase {
       const int v = std::atoi(e);
       if (v >= 0 && v <= 64) rtx = v;
     }
+    if (const char* e = std::getenv("CGFDM_SURFACE_STREAM")) surf_overlap = std::atoi(e) != 0 ? 1 : 0;
     if (const char* e = std::getenv("CGFDM_ROW_ALIGN")) galign = std::atoi(e) != 0;
     tNX = d.nx + 2 * cg::H;
     NX = tNX;
@@ -1455,6 +1466,11 @@ struct AwenoSolver : SolverBase {
   }
 
   int step(int n) override {
+    if (surf_overlap && !sstream) {
+      HIPCHK(hipStreamCreateWithFlags(&sstream, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
     for (int s = 0; s < n; ++s) {
       size_t sp = 0;
       if (timing_on) {
@@ -1507,8 +1523,16 @@ struct AwenoSolver : SolverBase {
           up = next_pair(1);
           HIPCHK(hipEventRecord(ev[up].a, stream));
         }
-        aw::launch_update<R>(a, d.nx, d.ny, stream);
+        if (surf_overlap) {
+          HIPCHK(hipEventRecord(ev_fork, stream));
+          HIPCHK(hipStreamWaitEvent(sstream, ev_fork, 0));
+        }
+        aw::launch_update<R>(a, d.nx, d.ny, stream, surf_overlap ? sstream : stream);
         if (timing_on) HIPCHK(hipEventRecord(ev[up].b, stream));
+        if (surf_overlap) {
+          HIPCHK(hipEventRecord(ev_join, sstream));
+          HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
+        }
       }
       cur = o1;
       W = buf[cur];
