@@ -8,6 +8,8 @@
 // 0..8 cofactors J*xi_x .. J*zt_z, 9 J, 10 mu, 11 lambda, 12 1/rho,
 // 13..15 x_xi, 16..18 x_eta.
 #pragma once
+#include <algorithm>
+
 #include "aweno.h"
 
 namespace aw {
@@ -475,14 +477,42 @@ __global__ void __launch_bounds__(256) aw_char_surface(R* __restrict__ W, const 
 // Exponential sponge (exp_decay_absorb.cpp:20-106): 20 layers, alpha 0.015,
 // factors of the sides a cell lies in applied one after another (x-low,
 // x-high, y-low, y-high, z-low, z-high without free surface).
+// The sponge frame is launched in two parts: whole 64 x 4 tiles of the planes
+// k0 .. k0 + gridDim.z - 1 (slab = 1: the bottom planes, and the top ones
+// without a free surface), and in the planes between them only the tiles that
+// reach the x or y frame (slab = 0): tile columns [0, xa) and [xb, ntx) in
+// every tile row, then tile rows [0, ya) and [yb, nty) of the other columns.
+constexpr int SPONGE_L = 20;
+
 template <class R>
 __global__ void __launch_bounds__(256) aw_sponge(R* __restrict__ W, int nx, int ny, int nz, int NX, long long N,
-                                                 long long S, int fs) {
-  const int I = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int Jj = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int K = blockIdx.z;
+                                                 long long S, int fs, int slab, int k0, int xa, int xb, int ya,
+                                                 int yb) {
+  const int ntx = (nx + 63) / 64, nty = (ny + 3) / 4;
+  int tx, ty;
+  if (slab) {
+    tx = blockIdx.x % ntx;
+    ty = blockIdx.x / ntx;
+  } else {
+    const int nxf = xa + (ntx - xb);  // frame tile columns
+    const int b = blockIdx.x;
+    if (b < nxf * nty) {
+      const int c = b / nty;
+      tx = c < xa ? c : xb + (c - xa);
+      ty = b % nty;
+    } else {
+      const int inner = xb - xa;  // > 0 here
+      const int b2 = b - nxf * nty;
+      const int r = b2 / inner;
+      tx = xa + b2 % inner;
+      ty = r < ya ? r : yb + (r - ya);
+    }
+  }
+  const int I = tx * 64 + (threadIdx.x & 63);
+  const int Jj = ty * 4 + (threadIdx.x >> 6);
+  const int K = k0 + blockIdx.z;
   if (I >= nx || Jj >= ny) return;
-  const int L = 20;
+  const int L = SPONGE_L;
   int pv[6];
   int n = 0;
   if (I <= L) pv[n++] = L - I;
@@ -609,7 +639,25 @@ void launch_char_surface(R* W, const R* cjm, int nx, int ny, int NX, int NZ, lon
 
 template <class R>
 void launch_sponge(R* W, int nx, int ny, int nz, int NX, long long N, long long S, int fs, hipStream_t st) {
-  aw_sponge<R><<<dim3((nx + 63) / 64, (ny + 3) / 4, nz), dim3(256), 0, st>>>(W, nx, ny, nz, NX, N, S, fs);
+  const int L = SPONGE_L;
+  const int ntx = (nx + 63) / 64, nty = (ny + 3) / 4;
+  // planes damped in z: K <= L (bottom), and K >= nz - L without a free surface
+  const int kb = std::min(L, nz - 1);
+  const int kt = fs ? nz : std::max(nz - L, kb + 1);  // first plane of the top slab
+  const dim3 block(256);
+  aw_sponge<R><<<dim3((unsigned)(ntx * nty), 1, (unsigned)(kb + 1)), block, 0, st>>>(W, nx, ny, nz, NX, N, S, fs, 1, 0,
+                                                                                   0, 0, 0, 0);
+  if (kt < nz)
+    aw_sponge<R><<<dim3((unsigned)(ntx * nty), 1, (unsigned)(nz - kt)), block, 0, st>>>(W, nx, ny, nz, NX, N, S, fs, 1,
+                                                                                      kt, 0, 0, 0, 0);
+  // planes kb + 1 .. kt - 1: tiles reaching I <= L or I >= nx - L, J <= L or J >= ny - L
+  const int nmid = kt - (kb + 1);
+  if (nmid <= 0) return;
+  int xa = std::min(ntx, L / 64 + 1), xb = std::max(xa, (nx - L) / 64);
+  int ya = std::min(nty, L / 4 + 1), yb = std::max(ya, (ny - L) / 4);
+  const int nblk = (xa + (ntx - xb)) * nty + (xb - xa) * (ya + (nty - yb));
+  aw_sponge<R><<<dim3((unsigned)nblk, 1, (unsigned)nmid), block, 0, st>>>(W, nx, ny, nz, NX, N, S, fs, 0, kb + 1, xa,
+                                                                        xb, ya, yb);
 }
 
 template <class R>
