@@ -9,8 +9,11 @@
  *   - int status codes instead of MPI_Abort (same numeric codes);
  *   - one GPU, one rank: the domain is the whole grid at its real shape, so
  *     the halo exchange (MPI_send_recv_FLOAT.cpp:388-505) disappears;
- *   - work is enqueued on one HIP stream (cgfdm_set_stream); calls are not
- *     thread safe on the same handle.
+ *   - work is enqueued on one HIP stream (cgfdm_set_stream); a step may fork
+ *     its surface-layer kernels onto an internal stream, joined back to that
+ *     stream with events before the step's later work, so everything a call
+ *     enqueues is complete once the solver stream is; calls are not thread
+ *     safe on the same handle.
  * Arrays are host memory unless stated otherwise. Grid arrays are SoA over the
  * padded grid, x fastest: a[c][k][j][i], NX = nx + 6 etc. (HALO = 3).
  */
