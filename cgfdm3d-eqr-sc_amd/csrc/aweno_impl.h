@@ -176,7 +176,10 @@ __device__ __forceinline__ void face_flux(const StageArgs<R>& a, unsigned o, uns
     for (int v = 0; v < 9; ++v) us[v] = up[v] + um[v];
     phys_flux<R>(us, n1, n2, n3, mu, lam, b, fs);
 #pragma unroll
-    for (int v = 0; v < 9; ++v) *at(F + v * N, o) = R(0.5) * (fs[v] - a.alpha * (up[v] - um[v]));
+    // non-temporal: the flux planes stream out and back in the update kernel
+    // without displacing the state rows the neighbouring faces re-read
+    for (int v = 0; v < 9; ++v)
+      __builtin_nontemporal_store(R(R(0.5) * (fs[v] - a.alpha * (up[v] - um[v]))), at(F + v * N, o));
   } else {
     R fm[9], fp[9];
     phys_flux<R>(um, n1, n2, n3, mu, lam, b, fm);
@@ -279,11 +282,13 @@ __device__ __forceinline__ void update_cell(const StageArgs<R>& a, int i, int j,
   R mv[9], wv[9];
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
-    mv[v] = pl(a.wm, N, v, o);
-    wv[v] = pl(a.win, N, v, o);
+    // the cell's own state values and its result stream through once:
+    // non-temporal, leaving the caches to the flux stencils
+    mv[v] = __builtin_nontemporal_load(at(a.wm + (long long)v * N, o));
+    wv[v] = __builtin_nontemporal_load(at(a.win + (long long)v * N, o));
   }
 #pragma unroll
-  for (int v = 0; v < 9; ++v) *at(a.wout + v * N, o) = tvd<R>(a.stage, mv[v], wv[v], h[v]);
+  for (int v = 0; v < 9; ++v) __builtin_nontemporal_store(tvd<R>(a.stage, mv[v], wv[v], h[v]), at(a.wout + v * N, o));
 }
 
 // One thread per interior column: 64-column tiles start at the first interior
