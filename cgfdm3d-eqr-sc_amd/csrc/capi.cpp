@@ -938,35 +938,50 @@ struct Solver : SolverBase {
     }
     return CGFDM_OK;
   }
-  // frees the checkpoints and work buffers of the last forward_record/adjoint;
-  // the stage store stays allocated for the next forward_record with the same
-  // checkpoint interval (it is the largest buffer; its allocation took up to a
-  // few seconds on the MI355X) and is freed with the solver
-  void release_adjoint() {
-    for (void* q : {(void*)ck_store, (void*)rec_vel, (void*)adj_buf, (void*)adj_face, (void*)adj_grad,
-                    (void*)adj_gsrc, (void*)adj_seg, (void*)adj_src, (void*)adj_end})
-      if (q) dfree(q);
-    ck_store = rec_vel = adj_buf = adj_face = adj_seg = adj_src = adj_end = nullptr;
-    adj_grad = adj_gsrc = nullptr;
-    fr_n = 0;
-    stages_seg = -1;
+  // (re)allocates *p for at least `count` elements; *cap is its capacity
+  template <class X>
+  int dgrow(X** p, long long* cap, long long count) {
+    if (*p && *cap >= count) return CGFDM_OK;
+    if (*p) dfree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (int rc = dalloc(p, count)) return rc;
+    *cap = count;
+    return CGFDM_OK;
   }
-  // bytes of the adjoint's work buffers besides the stage store
-  double adjoint_work_bytes(int nsteps, int ck) const {
+  long long ck_cap = 0, rec_cap = 0, seg_cap = 0, src_cap = 0;
+  // the checkpoints and the adjoint's work buffers for a forward_record of
+  // nsteps steps with a checkpoint every ck steps. forward_record allocates
+  // them before the stage store, which gets the memory that is left; they stay
+  // allocated with the solver (grown when a later run needs more), as the stage
+  // store does: a hipMalloc of a few GB took over a second on the MI355X where
+  // another process had freed the memory shortly before
+  int alloc_adjoint(int nsteps, int ck) {
+    const long long ss = state_size();
     long long fcnt = 0;
     for (int f = 0; f < 6; ++f) fcnt += 9 * pml.f[f].cnt;
-    const double r = (double)sizeof(R);
-    return r * (99.0 * N + 4.0 * std::max(fcnt, 1LL) + (double)state_size() * (ck + 1) +
-                3.0 * std::max(ns, 1) * nsteps) +
-           8.0 * (3.0 * N + 6.0 * std::max((long long)src_np * src_nt, 1LL));
+    const long long nrec = (long long)std::max(ns, 1) * nsteps * 3;
+    int rc;
+    if ((rc = dgrow(&ck_store, &ck_cap, ss * ((nsteps + ck - 1) / ck)))) return rc;
+    if ((rc = dgrow(&rec_vel, &rec_cap, nrec))) return rc;
+    if ((rc = dgrow(&adj_seg, &seg_cap, ss * ck))) return rc;
+    if ((rc = dgrow(&adj_src, &src_cap, nrec))) return rc;
+    if (!adj_buf) {
+      if ((rc = dalloc(&adj_buf, 99 * N))) return rc;
+      if ((rc = dalloc(&adj_face, std::max(4 * fcnt, 1LL)))) return rc;
+      if ((rc = dalloc(&adj_grad, 3 * N))) return rc;
+      if ((rc = dalloc(&adj_gsrc, std::max((long long)src_np * src_nt * 6, 1LL)))) return rc;
+      if ((rc = dalloc(&adj_end, ss))) return rc;
+    }
+    return CGFDM_OK;
   }
   // allocate the stage store of a ck-step segment if it fits the budget:
-  // CGFDM_ADJOINT_STORE_GB, else 85 % of the free device memory less `reserve`
-  // bytes still to be allocated (an MI355X's 288 GB hold the stage states of a
+  // CGFDM_ADJOINT_STORE_GB, else 85 % of the free device memory, allocated
+  // after the adjoint's work buffers (an MI355X's 288 GB hold the stage states of a
   // 100-step segment of a 256 x 256 x 128 grid). The store is optional: on a
   // failed allocation the runtime's sticky error is cleared so that the launch
   // checks of the recompute path do not report it.
-  void alloc_stage_store(int ck, double reserve) {
+  void alloc_stage_store(int ck) {
     const double need = (double)sizeof(R) * 36.0 * (double)N * ck;
     const char* env = std::getenv("CGFDM_ADJOINT_STORE_GB");
     const double env_budget = env ? std::atof(env) * 1073741824.0 : 0.0;
@@ -980,7 +995,7 @@ struct Solver : SolverBase {
     double budget = env_budget;
     if (!env) {
       size_t fre = 0, tot = 0;
-      if (hipMemGetInfo(&fre, &tot) == hipSuccess) budget = 0.85 * ((double)fre - reserve);
+      if (hipMemGetInfo(&fre, &tot) == hipSuccess) budget = 0.85 * (double)fre;
       (void)hipGetLastError();
     }
     if (need <= budget && dalloc(&adj_stages, 36LL * N * ck) != CGFDM_OK) {
@@ -1078,17 +1093,17 @@ struct Solver : SolverBase {
     if (!kCgfdmAdjointEnabled) return fail(CGFDM_ERR_DISABLED, kCgfdmAdjointOff);
     if (nsteps < 1 || ck < 1) return fail(CGFDM_ERR_ARG, "nsteps and checkpoint_every must be positive");
     if (d.coef_storage != 0) return fail(CGFDM_ERR_ARG, "the adjoint needs native coefficient storage");
-    release_adjoint();
+    fr_n = 0;
+    stages_seg = -1;
     int rc;
     const long long ss = state_size();
     const int nck = (nsteps + ck - 1) / ck;
-    if ((rc = dalloc(&ck_store, ss * nck))) return rc;
-    if ((rc = dalloc(&rec_vel, (long long)std::max(ns, 1) * nsteps * 3))) return rc;
+    if ((rc = alloc_adjoint(nsteps, ck))) return rc;
     fr_it0 = it;
     fr_ck = ck;
     // the last segment runs into the stage store when it fits, so that the
     // adjoint starts from its stage states instead of recomputing them
-    alloc_stage_store(ck, adjoint_work_bytes(nsteps, ck));
+    alloc_stage_store(ck);
     const int last0 = (nck - 1) * ck;
     for (int n = 0; n < nsteps; ++n) {
       if (n % ck == 0 && (rc = copy_state(ck_store + (long long)(n / ck) * ss, true))) return rc;
@@ -1123,15 +1138,6 @@ struct Solver : SolverBase {
     const long long ss = state_size();
     long long fcnt = 0;
     for (int f = 0; f < 6; ++f) fcnt += 9 * pml.f[f].cnt;
-    if (!adj_buf) {
-      if ((rc = dalloc(&adj_buf, 99 * N))) return rc;
-      if ((rc = dalloc(&adj_face, std::max(4 * fcnt, 1LL)))) return rc;
-      if ((rc = dalloc(&adj_grad, 3 * N))) return rc;
-      if ((rc = dalloc(&adj_gsrc, std::max((long long)src_np * src_nt * 6, 1LL)))) return rc;
-      if ((rc = dalloc(&adj_seg, ss * fr_ck))) return rc;
-      if ((rc = dalloc(&adj_src, (long long)std::max(ns, 1) * fr_n * 3))) return rc;
-      if ((rc = dalloc(&adj_end, ss))) return rc;
-    }
     R* MU = adj_buf;
     R* OUT = adj_buf + 9 * N;
     R* G = adj_buf + 18 * N;
@@ -1159,7 +1165,7 @@ struct Solver : SolverBase {
         off += 4 * c;
       }
     }
-    alloc_stage_store(fr_ck, 0.0);
+    alloc_stage_store(fr_ck);
     if ((rc = copy_state(adj_end, true))) return rc;
     const int nck = (fr_n + fr_ck - 1) / fr_ck;
     for (int seg = nck - 1; seg >= 0; --seg) {
