@@ -161,11 +161,14 @@ int cgfdm_get_ranges(cgfdm_solver* s, double* out8);
  * rec[ns][nsteps][3] in the wavefield precision. */
 int cgfdm_forward_record(cgfdm_solver* s, int nsteps, int checkpoint_every, double* rec);
 /* cgfdm_adjoint: discrete adjoint of the last cgfdm_forward_record for the adjoint source
- * src[ns][nsteps][3] = dJ/d rec. Segments between checkpoints are recomputed, the RK4 stages
- * of every step are recomputed from its start, and the transposed stages run backwards.
- * Outputs (any may be NULL): dJ/dmu, dJ/dlambda, dJ/d(1/rho) per padded cell
+ * src[ns][nsteps][3] = dJ/d rec. Segments between checkpoints are recomputed (the last one,
+ * and each segment's RK4 stage states, are kept in device memory when they fit the budget,
+ * else the stages of every step are recomputed from its start), and the transposed stages
+ * run backwards. Outputs (any may be NULL): dJ/dmu, dJ/dlambda, dJ/d(1/rho) per padded cell
  * (double[NZ][NY][NX]), and dJ/d(table given to cgfdm_set_source) as double[npts][nt][6].
- * The solver is left at the end of the forward run. */
+ * The solver is left at the end of the forward run. The checkpoints, the stage states and
+ * the adjoint's work buffers stay allocated until cgfdm_destroy (reused, and grown when
+ * needed, by the next cgfdm_forward_record). */
 int cgfdm_adjoint(cgfdm_solver* s, const double* src, double* gmu, double* glam, double* gb, double* grate);
 
 /* Per-kernel timing with HIP events on the solver stream. enable = 1 starts
