@@ -950,38 +950,42 @@ struct Solver : SolverBase {
     return CGFDM_OK;
   }
   long long ck_cap = 0, rec_cap = 0, seg_cap = 0, src_cap = 0;
-  // the checkpoints and the adjoint's work buffers for a forward_record of
-  // nsteps steps with a checkpoint every ck steps. forward_record allocates
-  // them before the stage store, which gets the memory that is left; they stay
-  // allocated with the solver (grown when a later run needs more), as the stage
-  // store does: a hipMalloc of a few GB took over a second on the MI355X where
-  // another process had freed the memory shortly before
-  int alloc_adjoint(int nsteps, int ck) {
-    const long long ss = state_size();
+  // the adjoint's work buffers, allocated by the first adjoint and kept with
+  // the solver, as the checkpoints and the stage store are (reused, and grown
+  // when a later run needs more): a hipMalloc of a few GB took over a second on
+  // the MI355X where another process had freed the memory shortly before
+  int alloc_adjoint_work() {
+    if (adj_buf) return CGFDM_OK;
     long long fcnt = 0;
     for (int f = 0; f < 6; ++f) fcnt += 9 * pml.f[f].cnt;
-    const long long nrec = (long long)std::max(ns, 1) * nsteps * 3;
     int rc;
-    if ((rc = dgrow(&ck_store, &ck_cap, ss * ((nsteps + ck - 1) / ck)))) return rc;
-    if ((rc = dgrow(&rec_vel, &rec_cap, nrec))) return rc;
-    if ((rc = dgrow(&adj_seg, &seg_cap, ss * ck))) return rc;
-    if ((rc = dgrow(&adj_src, &src_cap, nrec))) return rc;
-    if (!adj_buf) {
-      if ((rc = dalloc(&adj_buf, 99 * N))) return rc;
-      if ((rc = dalloc(&adj_face, std::max(4 * fcnt, 1LL)))) return rc;
-      if ((rc = dalloc(&adj_grad, 3 * N))) return rc;
-      if ((rc = dalloc(&adj_gsrc, std::max((long long)src_np * src_nt * 6, 1LL)))) return rc;
-      if ((rc = dalloc(&adj_end, ss))) return rc;
+    if ((rc = dalloc(&adj_buf, 99 * N)) || (rc = dalloc(&adj_face, std::max(4 * fcnt, 1LL))) ||
+        (rc = dalloc(&adj_grad, 3 * N)) || (rc = dalloc(&adj_gsrc, std::max((long long)src_np * src_nt * 6, 1LL))) ||
+        (rc = dalloc(&adj_end, state_size()))) {
+      // all or none, so that the next call allocates them again
+      for (void* q : {(void*)adj_buf, (void*)adj_face, (void*)adj_grad, (void*)adj_gsrc, (void*)adj_end})
+        if (q) dfree(q);
+      adj_buf = adj_face = adj_end = nullptr;
+      adj_grad = adj_gsrc = nullptr;
+      return rc;
     }
     return CGFDM_OK;
   }
+  // bytes of the work buffers above still to be allocated
+  double adjoint_work_bytes() const {
+    if (adj_buf) return 0.0;
+    long long fcnt = 0;
+    for (int f = 0; f < 6; ++f) fcnt += 9 * pml.f[f].cnt;
+    return (double)sizeof(R) * (99.0 * N + 4.0 * std::max(fcnt, 1LL) + (double)state_size()) +
+           8.0 * (3.0 * N + 6.0 * std::max((long long)src_np * src_nt, 1LL));
+  }
   // allocate the stage store of a ck-step segment if it fits the budget:
-  // CGFDM_ADJOINT_STORE_GB, else 85 % of the free device memory, allocated
-  // after the adjoint's work buffers (an MI355X's 288 GB hold the stage states of a
+  // CGFDM_ADJOINT_STORE_GB, else 85 % of the free device memory less `reserve`
+  // bytes still to be allocated (an MI355X's 288 GB hold the stage states of a
   // 100-step segment of a 256 x 256 x 128 grid). The store is optional: on a
   // failed allocation the runtime's sticky error is cleared so that the launch
   // checks of the recompute path do not report it.
-  void alloc_stage_store(int ck) {
+  void alloc_stage_store(int ck, double reserve) {
     const double need = (double)sizeof(R) * 36.0 * (double)N * ck;
     const char* env = std::getenv("CGFDM_ADJOINT_STORE_GB");
     const double env_budget = env ? std::atof(env) * 1073741824.0 : 0.0;
@@ -995,7 +999,7 @@ struct Solver : SolverBase {
     double budget = env_budget;
     if (!env) {
       size_t fre = 0, tot = 0;
-      if (hipMemGetInfo(&fre, &tot) == hipSuccess) budget = 0.85 * (double)fre;
+      if (hipMemGetInfo(&fre, &tot) == hipSuccess) budget = 0.85 * ((double)fre - reserve);
       (void)hipGetLastError();
     }
     if (need <= budget && dalloc(&adj_stages, 36LL * N * ck) != CGFDM_OK) {
@@ -1098,12 +1102,13 @@ struct Solver : SolverBase {
     int rc;
     const long long ss = state_size();
     const int nck = (nsteps + ck - 1) / ck;
-    if ((rc = alloc_adjoint(nsteps, ck))) return rc;
+    if ((rc = dgrow(&ck_store, &ck_cap, ss * nck))) return rc;
+    if ((rc = dgrow(&rec_vel, &rec_cap, (long long)std::max(ns, 1) * nsteps * 3))) return rc;
     fr_it0 = it;
     fr_ck = ck;
     // the last segment runs into the stage store when it fits, so that the
     // adjoint starts from its stage states instead of recomputing them
-    alloc_stage_store(ck);
+    alloc_stage_store(ck, adjoint_work_bytes());
     const int last0 = (nck - 1) * ck;
     for (int n = 0; n < nsteps; ++n) {
       if (n % ck == 0 && (rc = copy_state(ck_store + (long long)(n / ck) * ss, true))) return rc;
@@ -1138,6 +1143,8 @@ struct Solver : SolverBase {
     const long long ss = state_size();
     long long fcnt = 0;
     for (int f = 0; f < 6; ++f) fcnt += 9 * pml.f[f].cnt;
+    if ((rc = alloc_adjoint_work())) return rc;
+    if ((rc = dgrow(&adj_src, &src_cap, (long long)std::max(ns, 1) * fr_n * 3))) return rc;
     R* MU = adj_buf;
     R* OUT = adj_buf + 9 * N;
     R* G = adj_buf + 18 * N;
@@ -1165,7 +1172,9 @@ struct Solver : SolverBase {
         off += 4 * c;
       }
     }
-    alloc_stage_store(fr_ck);
+    alloc_stage_store(fr_ck, 0.0);
+    // without the stage store the segment's step states are kept instead
+    if (!adj_stages && (rc = dgrow(&adj_seg, &seg_cap, ss * fr_ck))) return rc;
     if ((rc = copy_state(adj_end, true))) return rc;
     const int nck = (fr_n + fr_ck - 1) / fr_ck;
     for (int seg = nck - 1; seg >= 0; --seg) {
