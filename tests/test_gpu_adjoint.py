@@ -144,6 +144,40 @@ def test_stage_store_reused_by_the_next_forward_record(solver_mod):
         np.testing.assert_allclose(grads[k], grads_h[k], rtol=0, atol=1e-12 * scale, err_msg=k)
 
 
+@pytest.mark.parametrize("store_gb", [None, "0"])
+def test_adjoint_buffers_reused_across_run_lengths(store_gb, solver_mod, monkeypatch):
+    """The checkpoints, station records and work buffers of the adjoint stay
+    allocated with the solver and grow when a later forward_record needs more
+    (more steps, more checkpoints, a longer segment) or are reused when it needs
+    less. Runs of 6, 10 and 4 steps with checkpoints every 4, 5 and 3 steps on
+    one solver give the gradients of fresh solvers started at the same steps,
+    with the stage store (None: sized from the free memory) and without it
+    ("0": the segment states are recomputed)."""
+    if store_gb is not None:
+        monkeypatch.setenv("CGFDM_ADJOINT_STORE_GB", store_gb)
+    runs = [(6, 4), (10, 5), (4, 3)]
+    m = M.small_config(nt=sum(n for n, _ in runs) + 1, **CASES["pml_fs_curvilinear"])
+    print(f"adjoint buffer reuse over runs {runs}, store budget {store_gb}", flush=True)
+    g = solver_mod.Solver(m, precision=8)
+    start = 0
+    for nst, ck in runs:
+        rec = g.forward_record(nst, ck)
+        grads = g.adjoint(rec)
+        h = solver_mod.Solver(m, precision=8)
+        if start:
+            h.step(start)
+        rec_h = h.forward_record(nst, ck)
+        grads_h = h.adjoint(rec_h)
+        h.close()
+        np.testing.assert_array_equal(rec, rec_h)
+        for k in ("gmu", "glam", "gb", "grate"):
+            scale = np.abs(grads_h[k]).max()
+            assert scale > 0, (nst, ck, k)
+            np.testing.assert_allclose(grads[k], grads_h[k], rtol=0, atol=1e-12 * scale, err_msg=f"{nst} {ck} {k}")
+        start += nst
+    g.close()
+
+
 def test_autograd_wrapper_on_the_hip_backend(oracle_mod):
     """torch gradients through the HIP adjoint equal those through the oracle."""
     import torch
