@@ -182,8 +182,9 @@ struct SolverBase {
   double ranges[8] = {0};
   long long bytes = 0;
 
-  // device allocations owned by the solver, zero-filled
+  // device allocations owned by the solver, zero-filled, and their sizes
   std::vector<void*> allocs;
+  std::vector<size_t> alloc_bytes;
   template <class X>
   int dalloc(X** p, long long count) {
     size_t nb = (size_t)count * sizeof(X);
@@ -192,8 +193,12 @@ struct SolverBase {
     hipError_t e = hipMalloc(&q, nb);
     if (e != hipSuccess) return fail(CGFDM_ERR_ALLOC, "hipMalloc of " + std::to_string(nb) + " bytes failed");
     e = hipMemsetAsync(q, 0, nb, stream);
-    if (e != hipSuccess) return fail(CGFDM_ERR_HIP, "hipMemset failed");
+    if (e != hipSuccess) {
+      (void)hipFree(q);  // the memset's error is the one reported
+      return fail(CGFDM_ERR_HIP, "hipMemset failed");
+    }
     allocs.push_back(q);
+    alloc_bytes.push_back(nb);
     bytes += (long long)nb;
     *p = (X*)q;
     return CGFDM_OK;
@@ -219,7 +224,9 @@ struct SolverBase {
     for (size_t q = 0; q < allocs.size(); ++q)
       if (allocs[q] == p) {
         (void)hipFree(p);  // a failed free leaves nothing to recover
+        bytes -= (long long)alloc_bytes[q];
         allocs.erase(allocs.begin() + q);
+        alloc_bytes.erase(alloc_bytes.begin() + q);
         return;
       }
   }

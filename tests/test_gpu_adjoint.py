@@ -178,6 +178,29 @@ def test_adjoint_buffers_reused_across_run_lengths(store_gb, solver_mod, monkeyp
     g.close()
 
 
+def test_device_bytes_drop_when_the_stage_store_is_freed(solver_mod, monkeypatch):
+    """The solver's device-memory count goes down by what it frees: a
+    forward_record under a zero stage-store budget frees the store that the
+    previous one allocated (4 stage states of 9 fields per step of the 3-step
+    segment, FP64: at least 36 x 3 x 8 bytes per cell)."""
+    monkeypatch.delenv("CGFDM_ADJOINT_STORE_GB", raising=False)
+    nst, ck = 6, 3
+    m = M.small_config(nt=2 * nst + 1, **CASES["pml_fs_curvilinear"])
+    print(f"device bytes with and without the stage store: {nst} steps, checkpoint {ck}", flush=True)
+    g = solver_mod.Solver(m, precision=8)
+    g.adjoint(g.forward_record(nst, ck))
+    with_store = g.device_bytes()
+    monkeypatch.setenv("CGFDM_ADJOINT_STORE_GB", "0")
+    rec = g.forward_record(nst, ck)
+    without = g.device_bytes()
+    g.adjoint(rec)  # the recompute path then keeps the segment's step states
+    assert g.device_bytes() > without
+    cells = g.N
+    g.close()
+    assert without > 0
+    assert with_store - without >= 36 * ck * 8 * cells, (with_store, without)
+
+
 def test_autograd_wrapper_on_the_hip_backend(oracle_mod):
     """torch gradients through the HIP adjoint equal those through the oracle."""
     import torch
